@@ -1,0 +1,335 @@
+"""Python view of the alglin_amd C ABI (include/alglin_amd.h) via ctypes.
+
+`BorderedCR` mirrors alglin::BorderedCR<T> (src/Alglin/BABD_BorderedCR.hxx)
+method by method; compute runs in libalglin_amd.so on the GPU.  There is no
+CPU fallback: a missing library raises, and calls on a machine without a
+GPU fail inside the library.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libalglin_amd.so")
+_lib = None
+
+
+class AlglinError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libalglin_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AlglinError(f"{LIB_PATH} is missing: run `python -m alglin_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, ci = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int
+        cs = ctypes.c_char_p
+        sig = {
+            "alglin_bcr_create": (vp, [ci]),
+            "alglin_bcr_destroy": (None, [vp]),
+            "alglin_bcr_allocate": (ci, [vp] + [i32] * 6),
+            "alglin_bcr_select": (ci, [vp, cs, cs, cs]),
+            "alglin_bcr_dims": (ci, [vp, vp]),
+            "alglin_bcr_load": (ci, [vp, cs, i32, vp, i32]),
+            "alglin_bcr_load_bottom": (ci, [vp, vp, i32, vp, i32, vp, i32, vp, i32]),
+            "alglin_bcr_load_bottom2": (ci, [vp, vp, i32, vp, i32, vp, i32, vp, i32]),
+            "alglin_bcr_load_packed": (ci, [vp] * 8),
+            "alglin_bcr_factorize": (ci, [vp]),
+            "alglin_bcr_solve": (ci, [vp, vp]),
+            "alglin_bcr_solve_n": (ci, [vp, i32, vp, i32]),
+            "alglin_bcr_mv": (ci, [vp, vp, vp]),
+            "alglin_bcr_sparse_nnz": (i32, [vp]),
+            "alglin_bcr_sparse_pattern": (ci, [vp, vp, vp, i32]),
+            "alglin_bcr_sparse_values": (ci, [vp, vp]),
+            "alglin_bcr_sparse_load": (ci, [vp, vp, vp, i32, vp, i32, i32, ci]),
+            "alglin_bcr_check_matrix": (ci, [vp]),
+            "alglin_bcr_last_error": (cs, [vp]),
+            "alglin_bcr_info": (cs, [vp]),
+            "alglin_bcr_bind_device": (ci, [vp] * 8),
+            "alglin_bcr_factorize_device": (ci, [vp, ci]),
+            "alglin_bcr_solve_device": (ci, [vp, vp, i32, i64, ci]),
+            "alglin_bcr_residual_device": (ci, [vp, vp, vp, vp, ci]),
+            "alglin_bcr_sync": (ci, [vp]),
+            "alglin_bcr_stream": (vp, [vp]),
+            "alglin_bcr_get_factors": (ci, [vp] * 9),
+            "alglin_bcr_fast_path": (ci, [vp]),
+            "alglin_bcr_set_timing": (ci, [vp, ci]),
+            "alglin_bcr_timing": (ci, [vp, vp, ci]),
+            "alglin_dev_malloc": (vp, [i64]),
+            "alglin_dev_free": (ci, [vp]),
+            "alglin_memcpy_htod": (ci, [vp, vp, i64]),
+            "alglin_memcpy_dtoh": (ci, [vp, vp, i64]),
+            "alglin_memcpy_dtod": (ci, [vp, vp, i64]),
+            "alglin_device_synchronize": (ci, []),
+            "alglin_device_info": (ci, [ctypes.c_char_p, i32, vp, vp]),
+            "BABD_factorize": (ci, [i32] * 7 + [vp, i32] * 4),
+            "BABD_factorize_bordered": (ci, [i32] * 9 + [vp, i32] * 7),
+            "BABD_solve": (ci, [i32, vp]),
+            "BABD_get_last_error": (cs, []),
+            "BABD_free": (ci, [i32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class BorderedCR:
+    """GPU BorderedCR<double|float> with the reference's method names."""
+
+    def __init__(self, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self._h = lib().alglin_bcr_create(0 if self.dtype == np.float64 else 1)
+        if not self._h:
+            raise AlglinError("alglin_bcr_create failed (no HIP device?)")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().alglin_bcr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise AlglinError(f"{what}: {self.last_error()}")
+
+    # --- setup ----------------------------------------------------------------
+    def allocate(self, N, n, qr, qx, nr, nx):
+        self._check(lib().alglin_bcr_allocate(self._h, N, n, qr, qx, nr, nx), "allocate")
+
+    def select(self, internal="LU", last="LU", last2="NONE"):
+        self._check(lib().alglin_bcr_select(self._h, internal.encode(), last.encode(), last2.encode()), "select")
+
+    def dims(self):
+        out = np.zeros(10, np.int32)
+        self._check(lib().alglin_bcr_dims(self._h, _p(out)), "dims")
+        keys = ("N", "n", "qr", "qx", "nr", "nx", "nrows", "ncols", "Nr", "Nc")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def nrows(self):
+        return self.dims()["nrows"]
+
+    def ncols(self):
+        return self.dims()["ncols"]
+
+    def last_error(self) -> str:
+        return lib().alglin_bcr_last_error(self._h).decode()
+
+    def info(self) -> str:
+        return lib().alglin_bcr_info(self._h).decode()
+
+    def load(self, what: str, nbl: int, A: np.ndarray):
+        """Reference loaders: what in B,C,D,E,DE,DEB,F,Cq,CqF,H,add_B,add_C,add_C2,add_C2F,add_F."""
+        A = np.asfortranarray(A, dtype=self.dtype)
+        self._check(lib().alglin_bcr_load(self._h, what.encode(), nbl, _p(A), max(1, A.shape[0])), f"load_{what}")
+
+    def load_bottom(self, H0, HN, Hq, Hp):
+        arr = [np.asfortranarray(a, dtype=self.dtype) for a in (H0, HN, Hq, Hp)]
+        args = []
+        for a in arr:
+            args += [_p(a), max(1, a.shape[0])]
+        self._check(lib().alglin_bcr_load_bottom(self._h, *args), "load_bottom")
+
+    def load_bottom2(self, C0, CN, Cq, F):
+        arr = [np.asfortranarray(a, dtype=self.dtype) for a in (C0, CN, Cq, F)]
+        args = []
+        for a in arr:
+            args += [_p(a), max(1, a.shape[0])]
+        self._check(lib().alglin_bcr_load_bottom2(self._h, *args), "load_bottom2")
+
+    def load_system(self, s: dict):
+        """All blocks of a system dict (alglin_amd.systems layout)."""
+        self.allocate(s["N"], s["n"], s["qr"], s["qx"], s["nr"], s["nx"])
+        packed = packed_blocks(s, self.dtype)
+        self._keep = packed
+        self._check(lib().alglin_bcr_load_packed(self._h, *[_p(packed[k]) for k in
+                                                           ("D", "E", "B", "C", "H", "Cq", "F")]), "load_packed")
+
+    # --- compute --------------------------------------------------------------
+    def factorize(self) -> bool:
+        return lib().alglin_bcr_factorize(self._h) == 0
+
+    def solve(self, b: np.ndarray):
+        """One rhs (nrows) -> (ok, x[ncols]); in place on a max(nrows,ncols) buffer."""
+        d = self.dims()
+        L = max(d["nrows"], d["ncols"])
+        x = np.zeros(L, self.dtype)
+        x[: len(b)] = b
+        ok = lib().alglin_bcr_solve(self._h, _p(x)) == 0
+        return ok, x[: d["ncols"]]
+
+    def solve_n(self, Bm: np.ndarray):
+        d = self.dims()
+        L = max(d["nrows"], d["ncols"])
+        nrhs = Bm.shape[1]
+        X = np.zeros((L, nrhs), self.dtype, order="F")
+        X[: Bm.shape[0]] = Bm
+        ok = lib().alglin_bcr_solve_n(self._h, nrhs, _p(X), L) == 0
+        return ok, np.array(X[: d["ncols"]])
+
+    def Mv(self, x):
+        x = np.ascontiguousarray(x, self.dtype)
+        y = np.zeros(self.nrows(), self.dtype)
+        self._check(lib().alglin_bcr_mv(self._h, _p(x), _p(y)), "Mv")
+        return y
+
+    def sparse(self, offs=0):
+        nnz = lib().alglin_bcr_sparse_nnz(self._h)
+        I = np.zeros(nnz, np.int32)
+        J = np.zeros(nnz, np.int32)
+        V = np.zeros(nnz, self.dtype)
+        self._check(lib().alglin_bcr_sparse_pattern(self._h, _p(I), _p(J), offs), "sparse_pattern")
+        self._check(lib().alglin_bcr_sparse_values(self._h, _p(V)), "sparse_values")
+        return I, J, V
+
+    def sparse_load(self, V, I, J, r_offs=0, c_offs=0, do_error=True):
+        V = np.ascontiguousarray(V, self.dtype)
+        I = np.ascontiguousarray(I, np.int32)
+        J = np.ascontiguousarray(J, np.int32)
+        self._check(lib().alglin_bcr_sparse_load(self._h, _p(V), _p(I), r_offs, _p(J), c_offs, len(V),
+                                                int(do_error)), "sparse_load")
+
+    def check_matrix(self):
+        self._check(lib().alglin_bcr_check_matrix(self._h), "check_matrix")
+
+    def factors(self):
+        d = self.dims()
+        N, n, nr, nx = d["N"], d["n"], d["nr"], d["nx"]
+        out = dict(
+            T=np.zeros(N * (2 * n * n + n), self.dtype), perm=np.zeros(N * n, np.int32),
+            D=np.zeros(N * n * n, self.dtype), E=np.zeros(N * n * n, self.dtype),
+            B=np.zeros(max(1, N * n * nx), self.dtype), C=np.zeros((N + 1) * nr * n, self.dtype),
+            last=np.zeros(d["Nr"] * d["Nc"]), pend=np.zeros(1),
+        )
+        self._check(lib().alglin_bcr_get_factors(self._h, *[_p(out[k]) for k in
+                                                           ("T", "perm", "D", "E", "B", "C", "last", "pend")]),
+                    "get_factors")
+        return out
+
+    def fast_path(self) -> bool:
+        return lib().alglin_bcr_fast_path(self._h) == 1
+
+    # --- device-resident (DeviceArray operands) --------------------------------
+    def bind_device(self, t: dict):
+        """t: dict of DeviceArray in packed layout (see packed_blocks / DeviceArray.from_host)."""
+        self._bound = t
+        ptr = [ctypes.c_void_p(t[k].data_ptr()) for k in ("D", "E", "B", "C", "H", "Cq", "F")]
+        self._check(lib().alglin_bcr_bind_device(self._h, *ptr), "bind_device")
+
+    def set_timing(self, on=True):
+        self._check(lib().alglin_bcr_set_timing(self._h, int(on)), "set_timing")
+
+    def timing(self, reset=True) -> dict:
+        out = np.zeros(8)
+        self._check(lib().alglin_bcr_timing(self._h, _p(out), int(reset)), "timing")
+        keys = ("pair_ms", "forward_ms", "backward_ms", "last_ms", "reduce_ms", "residual_ms",
+                "pair_launches", "sweep_launches")
+        return dict(zip(keys, (float(v) for v in out)))
+
+    def factorize_device(self, sync=True) -> bool:
+        return lib().alglin_bcr_factorize_device(self._h, int(sync)) == 0
+
+    def solve_device(self, X, nrhs: int, ld: int, sync=True):
+        self._check(lib().alglin_bcr_solve_device(self._h, ctypes.c_void_p(X.data_ptr()), nrhs, ld, int(sync)),
+                    "solve_device")
+
+    def residual_device(self, x, b, r, sync=True):
+        self._check(lib().alglin_bcr_residual_device(self._h, ctypes.c_void_p(x.data_ptr()),
+                                                    ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(r.data_ptr()),
+                                                    int(sync)), "residual_device")
+
+    def sync(self):
+        self._check(lib().alglin_bcr_sync(self._h), "sync")
+
+    def stream(self) -> int:
+        return lib().alglin_bcr_stream(self._h)
+
+
+class DeviceArray:
+    """A flat device allocation made through the library's own HIP runtime."""
+
+    def __init__(self, count: int, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = max(1, self.count * self.dtype.itemsize)
+        self.ptr = lib().alglin_dev_malloc(self.nbytes)
+        if not self.ptr:
+            raise AlglinError(f"device allocation of {self.nbytes} bytes failed")
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, dtype=None):
+        a = np.ascontiguousarray(a, dtype or a.dtype)
+        d = cls(a.size, a.dtype)
+        if a.size and lib().alglin_memcpy_htod(ctypes.c_void_p(d.ptr), _p(a), a.nbytes) != 0:
+            raise AlglinError("H2D copy failed")
+        return d
+
+    def data_ptr(self) -> int:
+        return self.ptr
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        if self.count and lib().alglin_memcpy_dtoh(_p(out), ctypes.c_void_p(self.ptr), out.nbytes) != 0:
+            raise AlglinError("D2H copy failed")
+        return out
+
+    def copy_from(self, other: "DeviceArray"):
+        if lib().alglin_memcpy_dtod(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr),
+                                     min(self.nbytes, other.nbytes)) != 0:
+            raise AlglinError("D2D copy failed")
+
+    def free(self):
+        if self.ptr:
+            lib().alglin_dev_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    if lib().alglin_device_synchronize() != 0:
+        raise AlglinError("hipDeviceSynchronize failed")
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(128)
+    cus = ctypes.c_int32(0)
+    hbm = ctypes.c_int64(0)
+    if lib().alglin_device_info(name, 128, ctypes.byref(cus), ctypes.byref(hbm)) != 0:
+        raise AlglinError("no HIP device")
+    return dict(arch=name.value.decode(), cus=cus.value, hbm_bytes=hbm.value)
+
+
+def packed_blocks(s: dict, dtype=np.float64) -> dict:
+    """System dict -> arrays in the reference's internal block layout (hxx:972-998)."""
+    dt = np.dtype(dtype)
+
+    def blocks(a):  # (K, rows, cols) -> K column-major blocks, contiguous
+        return np.ascontiguousarray(np.transpose(np.asarray(a, dt), (0, 2, 1))).reshape(-1)
+
+    def mat(a):
+        return np.ascontiguousarray(np.asarray(a, dt).T).reshape(-1)
+
+    return dict(D=blocks(s["D"]), E=blocks(s["E"]), B=blocks(s["B"]), C=blocks(s["C"]),
+                H=mat(s["H"]), Cq=mat(s["Cq"]), F=mat(s["F"]))

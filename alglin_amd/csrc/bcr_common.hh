@@ -1,0 +1,95 @@
+// Shared device-side definitions for the bordered cyclic-reduction kernels.
+//
+// Data layout in HBM (one system): the block arrays keep the block-contiguous,
+// column-major layout of the reference (BABD_BorderedCR.hxx:972-998):
+//   D_k, E_k : n x n      at D + k*n*n        (k = 0..N-1)
+//   B_k      : n x nx     at B + k*n*nx
+//   C_k      : nr x n     at C + k*nr*n       (k = 0..N)
+//   T_j      : 2n x n panel (L\U or Householder vectors) + n taus at Tm + j*(2n*n+n)
+//   Perm_j   : n ints (LU: 1-based ipiv; QRP: 0-based exchange list)
+// so a workgroup that owns a pair streams each block as one contiguous run.
+//
+// Cross-pair writes of the serial reference (pair j updates C_{j+k}, which
+// pair j+2k also updates as its C_jp, BABD_BorderedCR.cc:1124-1143; every
+// pair accumulates into F, cc:1146-1154) are made race-free with
+//   * two pending-C slots per block, Cpend[slot][x], written at level k by the
+//     pair whose jpp == x into slot log2(k)&1 and folded in by the pair that
+//     next reads C_x (see bcr_factor.hip for the proof of disjointness), and
+//   * one F partial per eliminated block, Fpart[j], summed after the levels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bcr {
+
+constexpr int kWave = 64;
+constexpr int kRhsChunk = 16;  // right-hand sides per pass of the sweep kernels
+
+enum Method : int { kLU = 0, kQR = 1, kQRP = 2 };
+
+// Per-level arguments of the pair-elimination kernels.
+template <typename TS, typename TW>
+struct FactorArgs {
+  int N, n, nr, nx;
+  int L;           // level stride k (1, 2, 4, ...)
+  int npairs;      // pairs at this level
+  int slot_cur;    // pending-C slot written at this level
+  int first;       // 1 at the first level (nothing pending to fold in yet)
+  const TS* sD; const TS* sE; const TS* sB; const TS* sC;  // sources
+  TW* D; TW* E; TW* B; TW* C;                              // working blocks
+  TW* Tm; int32_t* perm;
+  TW* Cpend;       // [2][N+1][nr*n]
+  TW* Fpart;       // [N][nr*nx]
+  int32_t* status; // set to 1 when a pivot/diagonal is exactly zero
+};
+
+// Per-level arguments of the forward / backward sweeps.
+template <typename TW, typename TX>
+struct SolveArgs {
+  int N, n, nr, nx, qr, qx;
+  int L, npairs;
+  int nrhs;
+  int64_t ldx;        // leading dimension of X
+  TX* x;              // column-major X (ldx x nrhs)
+  const TW* D; const TW* E; const TW* B; const TW* C;
+  const TW* Tm; const int32_t* perm;
+  TX* Xpart;          // [N][nr*nrhs]
+};
+
+// Arguments of the last-block assembly.
+template <typename TW, typename TA>
+struct AssembleArgs {
+  int N, n, qr, qx, nr, nx, Nr, Nc;
+  const TW* D0; const TW* E0; const TW* B0;   // working row 0
+  const TW* C;                                // working C (C_0 and C_N used)
+  TW const* Cpend;                            // [2][N+1][nr*n]
+  const TA* H; const TA* Cq;                  // pristine
+  const double* Feff;                         // nr x nx (pristine F + sum of partials)
+  double* Hm;                                 // Nr x Nc output
+};
+
+// Arguments of the FP64 block SpMV (pristine operator).
+template <typename TA>
+struct MvArgs {
+  int N, n, qr, qx, nr, nx;
+  const TA* D; const TA* E; const TA* B; const TA* C; const TA* H; const TA* Cq; const TA* F;
+  const double* x;     // ncols
+  const double* b;     // nrows (residual mode) or nullptr (y = A x)
+  double* out;         // nrows
+  double* Cpart;       // [N+1][nr]
+};
+
+__host__ __device__ __forceinline__ size_t tsize(int n) { return size_t(2) * n * n + n; }
+__host__ __device__ __forceinline__ size_t align16(size_t bytes) { return (bytes + 15) & ~size_t(15); }
+
+template <typename T> __device__ __forceinline__ T dev_abs(T v) { return v < T(0) ? -v : v; }
+
+// pair p of level L (pure cyclic reduction over [0,N), cc:1088-1133)
+__device__ __forceinline__ void pair_index(int p, int L, int N, int& j, int& jp, int& jpp) {
+  j = L + 2 * L * p;
+  jp = j - L;
+  jpp = min(j + L, N);
+}
+
+}  // namespace bcr

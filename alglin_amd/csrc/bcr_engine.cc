@@ -1,0 +1,322 @@
+// Launch sequences of the GPU engine (see bcr_engine.hh).
+#include "bcr_engine.hh"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bcr_launch.hh"
+
+namespace bcr {
+
+void hip_check(hipError_t e, char const* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+void DevBuf<T>::resize(size_t c) {
+  if (c <= count && p != nullptr) return;
+  release();
+  if (c == 0) return;
+  hip_check(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)), "hipMalloc");
+  count = c;
+}
+
+template <typename T>
+void DevBuf<T>::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  count = 0;
+}
+
+template struct DevBuf<double>;
+template struct DevBuf<float>;
+template struct DevBuf<int32_t>;
+template struct DevBuf<char>;
+
+template <typename TA, typename TW>
+Engine<TA, TW>::Engine() {
+  hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
+  int dev = 0, lds = 0, cus = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    lds_limit_ = size_t(lds);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+    max_chunks_ = 4 * cus;
+}
+
+template <typename TA, typename TW>
+Engine<TA, TW>::~Engine() {
+  for (auto& r : recs_) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : event_pool_) (void)hipEventDestroy(e);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+template <typename TA, typename TW>
+hipEvent_t Engine<TA, TW>::get_event() {
+  if (!event_pool_.empty()) {
+    hipEvent_t e = event_pool_.back();
+    event_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+template <typename TA, typename TW>
+template <typename F>
+void Engine<TA, TW>::timed(int phase, F&& f) {
+  if (!timing_) { f(); return; }
+  Rec r{phase, get_event(), get_event()};
+  hip_check(hipEventRecord(r.a, st_), "hipEventRecord");
+  f();
+  hip_check(hipEventRecord(r.b, st_), "hipEventRecord");
+  recs_.push_back(r);
+  if (phase == kPair) acc_[6] += 1;
+  if (phase == kForward || phase == kBackward) acc_[7] += 1;
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::timing(double out[8], bool reset) {
+  hip_check(hipStreamSynchronize(st_), "timing sync");
+  for (auto& r : recs_) {
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+    acc_[r.phase] += ms;
+    event_pool_.push_back(r.a);
+    event_pool_.push_back(r.b);
+  }
+  recs_.clear();
+  for (int i = 0; i < 8; ++i) out[i] = acc_[i];
+  if (reset)
+    for (double& v : acc_) v = 0;
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::allocate(Dims const& d) {
+  d_ = d;
+  const size_t N = size_t(d.N), n = size_t(d.n), nr = size_t(d.nr), nx = size_t(d.nx);
+  wD_.resize(std::max<size_t>(1, N * n * n));
+  wE_.resize(std::max<size_t>(1, N * n * n));
+  wB_.resize(std::max<size_t>(1, N * n * nx));
+  wC_.resize(std::max<size_t>(1, (N + 1) * nr * n));
+  Tm_.resize(std::max<size_t>(1, N * tsize(d.n)));
+  perm_.resize(std::max<size_t>(1, N * n));
+  Cpend_.resize(std::max<size_t>(1, 2 * (N + 1) * nr * n));
+  Fpart_.resize(std::max<size_t>(1, N * nr * nx));
+  Feff_.resize(std::max<size_t>(1, nr * nx));
+  Hm_.resize(size_t(d.Nr()) * d.Nc());
+  ipiv_.resize(std::max(d.Nr(), d.Nc()));
+  status_.resize(4);
+  tmpw_.resize(size_t(max_chunks_) * std::max<size_t>(1, nr * nx));
+  tmpd_.resize(size_t(max_chunks_) * std::max<size_t>(1, nr * kRhsChunk));
+  Cpart_.resize((N + 1) * std::max<size_t>(1, nr));
+  Csum_.resize(std::max<size_t>(1, nr));
+  hip_check(hipStreamSynchronize(st_), "allocate sync");
+  Lmax_ = 1;
+  while (Lmax_ < d.N) Lmax_ *= 2;
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::upload(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq,
+                            TA const* F) {
+  const size_t N = size_t(d_.N), n = size_t(d_.n), nr = size_t(d_.nr), nx = size_t(d_.nx);
+  auto up = [&](DevBuf<TA>& b, TA const* src, size_t cnt) {
+    b.resize(std::max<size_t>(1, cnt));
+    if (cnt) hip_check(hipMemcpyAsync(b.p, src, cnt * sizeof(TA), hipMemcpyHostToDevice, st_), "H2D");
+  };
+  up(aD_, D, N * n * n);
+  up(aE_, E, N * n * n);
+  up(aB_, B, N * n * nx);
+  up(aC_, C, (N + 1) * nr * n);
+  up(aH_, H, (n + d_.qr) * size_t(d_.Nc()));
+  up(aCq_, Cq, nr * d_.qx);
+  up(aF_, F, nr * nx);
+  pD_ = aD_.p; pE_ = aE_.p; pB_ = aB_.p; pC_ = aC_.p; pH_ = aH_.p; pCq_ = aCq_.p; pF_ = aF_.p;
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::bind_device(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq,
+                                 TA const* F) {
+  pD_ = D; pE_ = E; pB_ = B; pC_ = C; pH_ = H; pCq_ = Cq; pF_ = F;
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::level_factor(int L, bool first) {
+  const int N = d_.N;
+  const int npairs = (N + L - 1) / (2 * L);
+  int lg = 0;
+  for (int t = L; t > 1; t >>= 1) ++lg;
+  auto fill = [&](auto& a) {
+    a.N = N; a.n = d_.n; a.nr = d_.nr; a.nx = d_.nx;
+    a.L = L; a.npairs = npairs; a.slot_cur = lg & 1; a.first = first ? 1 : 0;
+    a.D = wD_.p; a.E = wE_.p; a.B = wB_.p; a.C = wC_.p;
+    a.Tm = Tm_.p; a.perm = perm_.p; a.Cpend = Cpend_.p; a.Fpart = Fpart_.p; a.status = status_.p;
+  };
+  auto run = [&](auto& a) {
+    using A = std::remove_reference_t<decltype(a)>;
+    (void)sizeof(A);
+    hipError_t e = hipSuccess;
+    if (method_ == kLU && (launch_pair_lu_split(a, st_, &e) || launch_pair_lu_fast(a, st_, &e))) {
+      stats_.fast_path = 1;
+      hip_check(e, "pair_lu_fast");
+      return;
+    }
+    const size_t bytes = size_t(2 * d_.n + d_.nr) * (3 * d_.n + d_.nx) * sizeof(TW);
+    if (bytes > lds_limit_) gscratch_.resize(size_t(npairs) * (bytes / sizeof(TW)));
+    hip_check(launch_pair_generic(method_, a, gscratch_.p, lds_limit_, st_), "pair_generic");
+  };
+  if (first) {
+    FactorArgs<TA, TW> a{};
+    fill(a);
+    a.sD = pD_; a.sE = pE_; a.sB = pB_; a.sC = pC_;
+    run(a);
+  } else {
+    FactorArgs<TW, TW> a{};
+    fill(a);
+    a.sD = wD_.p; a.sE = wE_.p; a.sB = wB_.p; a.sC = wC_.p;
+    run(a);
+  }
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::factorize_async() {
+  const int N = d_.N, n = d_.n, nr = d_.nr, nx = d_.nx;
+  const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
+  if (!pD_) throw HipError("factorize: no operator loaded");
+  if (d_.Nr() != d_.Nc()) throw HipError("factorize: the GPU last-block LU needs Nr == Nc (qr+nr == qx+nx)");
+  // per-pair LDS tables of the generic and sweep kernels hold up to 64 pivots
+  // and 16 border rows
+  if (n > 64 || nr > 16) throw HipError("factorize: block size n <= 64 and nr <= 16 are supported");
+  stats_ = FactorStats{};
+  hip_check(launch_convert<int32_t, int32_t>(nullptr, status_.p, 4, st_), "status");
+  // pending border contributions accumulate per factorization
+  timed(kReduce, [&] { hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending"); });
+  // blocks the first level does not touch: row N-1 when N is odd, C_N
+  if (N % 2 == 1) {
+    const size_t k = size_t(N - 1);
+    hip_check(launch_convert<TA, TW>(pD_ + k * nn, wD_.p + k * nn, int64_t(nn), st_), "copy D");
+    hip_check(launch_convert<TA, TW>(pE_ + k * nn, wE_.p + k * nn, int64_t(nn), st_), "copy E");
+    hip_check(launch_convert<TA, TW>(pB_ + k * bn, wB_.p + k * bn, int64_t(bn), st_), "copy B");
+    hip_check(launch_convert<TA, TW>(pC_ + k * rn, wC_.p + k * rn, int64_t(rn), st_), "copy C");
+  }
+  hip_check(launch_convert<TA, TW>(pC_ + size_t(N) * rn, wC_.p + size_t(N) * rn, int64_t(rn), st_), "copy CN");
+  int L = 1;
+  bool first = true;
+  while (L < N) {
+    timed(kPair, [&] { level_factor(L, first); });
+    first = false;
+    L *= 2;
+    ++stats_.levels;
+  }
+  // F_eff = F + sum_j Fpart[j]  (cc:823-829 / 1146-1154)
+  const int mF = nr * nx;
+  if (mF > 0)
+    timed(kReduce, [&] {
+      hip_check(launch_convert<TA, double>(pF_, Feff_.p, mF, st_), "F");
+      if (N > 1)
+        hip_check(launch_reduce<TW, double>(Fpart_.p, 1, N - 1, mF, tmpw_.p, max_chunks_, Feff_.p, mF, mF, st_),
+                  "reduce F");
+    });
+  AssembleArgs<TW, TA> as{};
+  as.N = N; as.n = n; as.qr = d_.qr; as.qx = d_.qx; as.nr = nr; as.nx = nx; as.Nr = d_.Nr(); as.Nc = d_.Nc();
+  as.D0 = wD_.p; as.E0 = wE_.p; as.B0 = wB_.p; as.C = wC_.p; as.Cpend = Cpend_.p;
+  as.H = pH_; as.Cq = pCq_; as.Feff = Feff_.p; as.Hm = Hm_.p;
+  if (N == 1) {  // no reduction level: row 0 and C_0 come straight from A
+    hip_check(launch_convert<TA, TW>(pD_, wD_.p, int64_t(nn), st_), "copy D0");
+    hip_check(launch_convert<TA, TW>(pE_, wE_.p, int64_t(nn), st_), "copy E0");
+    hip_check(launch_convert<TA, TW>(pB_, wB_.p, int64_t(bn), st_), "copy B0");
+    hip_check(launch_convert<TA, TW>(pC_, wC_.p, int64_t(rn), st_), "copy C0");
+  }
+  timed(kLast, [&] {
+    hip_check(launch_assemble_last(as, st_), "assemble last");
+    hip_check(launch_lu_last(Hm_.p, d_.Nc(), ipiv_.p, status_.p, lds_limit_, st_), "lu last");
+  });
+}
+
+template <typename TA, typename TW>
+int32_t Engine<TA, TW>::status_sync() {
+  int32_t s = 0;
+  hip_check(hipMemcpyAsync(&s, status_.p, sizeof(int32_t), hipMemcpyDeviceToHost, st_), "status D2H");
+  hip_check(hipStreamSynchronize(st_), "sync");
+  return s;
+}
+
+template <typename TA, typename TW>
+template <typename TX>
+void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
+  const int N = d_.N, n = d_.n, nr = d_.nr;
+  SolveArgs<TW, TX> a{};
+  a.N = N; a.n = n; a.nr = nr; a.nx = d_.nx; a.qr = d_.qr; a.qx = d_.qx;
+  a.nrhs = nrhs; a.ldx = ldx; a.x = x;
+  a.D = wD_.p; a.E = wE_.p; a.B = wB_.p; a.C = wC_.p; a.Tm = Tm_.p; a.perm = perm_.p;
+  const size_t m = size_t(nr) * nrhs;
+  TX* Xpart = nullptr;
+  TX* tmp = nullptr;
+  if (sizeof(TX) == sizeof(double)) {
+    Xpartd_.resize(std::max<size_t>(1, size_t(N) * m));
+    tmpd_.resize(std::max<size_t>(1, size_t(max_chunks_) * m));
+    Xpart = reinterpret_cast<TX*>(Xpartd_.p);
+    tmp = reinterpret_cast<TX*>(tmpd_.p);
+  } else {
+    Xpartw_.resize(std::max<size_t>(1, size_t(N) * m));
+    tmpw_.resize(std::max<size_t>(size_t(max_chunks_) * std::max<size_t>(1, size_t(nr) * d_.nx), size_t(max_chunks_) * m));
+    Xpart = reinterpret_cast<TX*>(Xpartw_.p);
+    tmp = reinterpret_cast<TX*>(tmpw_.p);
+  }
+  a.Xpart = Xpart;
+  int L = 1;
+  while (L < N) {
+    a.L = L;
+    a.npairs = (N + L - 1) / (2 * L);
+    timed(kForward, [&] { hip_check(launch_forward(method_, a, st_), "forward"); });
+    L *= 2;
+  }
+  if (N > 1 && m > 0)
+    timed(kReduce, [&] {
+      hip_check(launch_reduce<TX, TX>(Xpart, 1, N - 1, int(m), tmp, max_chunks_, x + size_t(N + 1) * n + d_.qr, nr,
+                                      ldx, st_),
+                "reduce xb");
+    });
+  timed(kLast, [&] {
+    hip_check(launch_last_solve<TX>(Hm_.p, ipiv_.p, N, n, d_.qr, d_.qx, nr, d_.nx, x, ldx, nrhs, lds_limit_, st_),
+              "last solve");
+  });
+  int k = Lmax_;
+  while ((k /= 2) > 0) {
+    a.L = k;
+    a.npairs = (N + k - 1) / (2 * k);
+    timed(kBackward, [&] { hip_check(launch_backward(method_, a, st_), "backward"); });
+  }
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::residual_async(double const* x, double const* b, double* r) {
+  MvArgs<TA> m{};
+  m.N = d_.N; m.n = d_.n; m.qr = d_.qr; m.qx = d_.qx; m.nr = d_.nr; m.nx = d_.nx;
+  m.D = pD_; m.E = pE_; m.B = pB_; m.C = pC_; m.H = pH_; m.Cq = pCq_; m.F = pF_;
+  m.x = x; m.b = b; m.out = r; m.Cpart = Cpart_.p;
+  tmpd_.resize(std::max<size_t>(tmpd_.count, size_t(max_chunks_) * std::max(1, d_.nr)));
+  timed(kResidual, [&] { hip_check(launch_mv(m, Csum_.p, tmpd_.p, max_chunks_, st_), "mv"); });
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::mv_async(double const* x, double* y) {
+  residual_async(x, nullptr, y);
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::nrm2_async(double const* v, int64_t len, double* out) {
+  tmpd_.resize(std::max<size_t>(tmpd_.count, size_t(max_chunks_)));
+  hip_check(launch_nrm2(v, len, tmpd_.p, max_chunks_, out, st_), "nrm2");
+}
+
+template class Engine<double, double>;
+template class Engine<float, float>;
+template class Engine<double, float>;
+template void Engine<double, double>::solve_async<double>(double*, int, int64_t);
+template void Engine<float, float>::solve_async<float>(float*, int, int64_t);
+template void Engine<double, float>::solve_async<double>(double*, int, int64_t);
+
+}  // namespace bcr

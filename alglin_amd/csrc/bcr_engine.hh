@@ -1,0 +1,140 @@
+// GPU engine of one bordered BABD system: device buffers and the launch
+// sequence of factorize / solve / residual.  No torch types; plain HIP.
+//
+//   TA  element type of the pristine operator (double, or float for the
+//       BorderedCR<float> surface)
+//   TW  element type of the cyclic-reduction factors (== TA for the plain
+//       path, float for the mixed-precision path with TA = double)
+//
+// The pristine blocks are never modified: the first reduction level reads
+// them and writes the factor arrays, later levels work in the factor arrays.
+// That keeps A intact for the FP64 residual of iterative refinement without
+// a copy (the reference overwrites D/E/B/C/F in place, cc:1099-1153).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bcr_common.hh"
+
+namespace bcr {
+
+struct Dims {
+  int N = 0, n = 0, qr = 0, qx = 0, nr = 0, nx = 0;
+  int Nr() const { return 2 * n + nr + qr; }
+  int Nc() const { return 2 * n + nx + qx; }
+  int64_t nrows() const { return int64_t(n) * (N + 1) + qr + nr; }
+  int64_t ncols() const { return int64_t(n) * (N + 1) + qx + nx; }
+  bool operator==(Dims const& o) const {
+    return N == o.N && n == o.n && qr == o.qr && qx == o.qx && nr == o.nr && nx == o.nx;
+  }
+};
+
+class HipError : public std::runtime_error {
+ public:
+  explicit HipError(std::string const& s) : std::runtime_error(s) {}
+};
+
+void hip_check(hipError_t e, char const* what);
+
+// A device buffer owned by the engine.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  void resize(size_t c);
+  void release();
+  ~DevBuf() { release(); }
+};
+
+struct FactorStats {
+  int levels = 0;
+  int fast_path = 0;   // 1 when the register-resident LU kernel ran
+};
+
+template <typename TA, typename TW>
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(Engine const&) = delete;
+  Engine& operator=(Engine const&) = delete;
+
+  void allocate(Dims const& d);
+  Dims const& dims() const { return d_; }
+  hipStream_t stream() const { return st_; }
+
+  void set_method(int m) { method_ = m; }
+  int method() const { return method_; }
+
+  // pristine operator: upload from host block arrays (reference layout)
+  void upload(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq, TA const* F);
+  // or use caller-owned device arrays as the pristine operator (no copy)
+  void bind_device(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq, TA const* F);
+
+  // enqueue the factorization on the engine stream (no host sync)
+  void factorize_async();
+  // status after factorize_async: 0 ok, 1 zero pivot in a pair, 2 singular last block
+  int32_t status_sync();
+
+  // in-place solve on device X (ldx x nrhs), X type TX = TA or double
+  template <typename TX> void solve_async(TX* x, int nrhs, int64_t ldx);
+
+  // r = b - A x (FP64, pristine operator); all device pointers
+  void residual_async(double const* x, double const* b, double* r);
+  // y = A x
+  void mv_async(double const* x, double* y);
+  // ||v||_2 of a device vector into *out (device)
+  void nrm2_async(double const* v, int64_t len, double* out);
+
+  // internals (device pointers) for tests
+  TW const* factor_T() const { return Tm_.p; }
+  int32_t const* factor_perm() const { return perm_.p; }
+  TW const* work_D() const { return wD_.p; }
+  TW const* work_E() const { return wE_.p; }
+  TW const* work_B() const { return wB_.p; }
+  TW const* work_C() const { return wC_.p; }
+  double const* last_block() const { return Hm_.p; }
+  TW const* pending() const { return Cpend_.p; }
+  size_t pending_count() const { return Cpend_.count; }
+  FactorStats const& stats() const { return stats_; }
+
+  // per-phase timing with HIP events on the engine stream (see alglin_amd.h)
+  enum Phase { kPair = 0, kForward = 1, kBackward = 2, kLast = 3, kReduce = 4, kResidual = 5 };
+  void set_timing(bool on) { timing_ = on; }
+  void timing(double out[8], bool reset);
+
+ private:
+  template <typename F> void timed(int phase, F&& f);
+  struct Rec { int phase; hipEvent_t a, b; };
+  bool timing_ = false;
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> event_pool_;
+  double acc_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t get_event();
+
+  void level_factor(int L, bool first);
+  Dims d_;
+  int method_ = kLU;
+  hipStream_t st_ = nullptr;
+  size_t lds_limit_ = 65536;
+  int max_chunks_ = 1024;
+  FactorStats stats_;
+  int Lmax_ = 1;
+  // pristine (owned copy or bound)
+  DevBuf<TA> aD_, aE_, aB_, aC_, aH_, aCq_, aF_;
+  TA const *pD_ = nullptr, *pE_ = nullptr, *pB_ = nullptr, *pC_ = nullptr, *pH_ = nullptr, *pCq_ = nullptr,
+           *pF_ = nullptr;
+  // factors / working
+  DevBuf<TW> wD_, wE_, wB_, wC_, Tm_, Cpend_, Fpart_, gscratch_, tmpw_;
+  DevBuf<int32_t> perm_, ipiv_, status_;
+  DevBuf<double> Feff_, Hm_, tmpd_, Cpart_, Csum_, Xpartd_;
+  DevBuf<TW> Xpartw_;
+  bool pend_clean_ = false;
+};
+
+}  // namespace bcr

@@ -1,0 +1,6 @@
+// Instantiation of the register-resident LU pair kernel for (double source, float factors).
+#include "bcr_factor_fast.inc"
+
+namespace bcr {
+template bool launch_pair_lu_fast<double, float>(FactorArgs<double, float> const&, hipStream_t, hipError_t*);
+}  // namespace bcr

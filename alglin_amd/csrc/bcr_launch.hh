@@ -1,0 +1,45 @@
+// Host-callable launchers of the device kernels (defined in the .hip files).
+#pragma once
+
+#include "bcr_common.hh"
+
+namespace bcr {
+
+template <typename TW, typename TX> struct SolveArgs;
+template <typename TW, typename TA> struct AssembleArgs;
+template <typename TA> struct MvArgs;
+
+template <typename TS, typename TW>
+hipError_t launch_pair_generic(int method, FactorArgs<TS, TW> const& a, TW* gscratch, size_t lds_limit,
+                               hipStream_t st);
+template <typename TS, typename TW>
+bool launch_pair_lu_fast(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
+bool launch_pair_lu_split(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
+
+template <typename TW, typename TX>
+hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
+template <typename TW, typename TX>
+hipError_t launch_backward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
+template <typename TP, typename TD>
+hipError_t launch_reduce(const TP* part, int first, int count, int m, TP* tmp, int max_chunks, TD* dst, int mrow,
+                         int64_t ldd, hipStream_t st);
+
+template <typename TW, typename TA>
+hipError_t launch_assemble_last(AssembleArgs<TW, TA> const& a, hipStream_t st);
+hipError_t launch_lu_last(double* Hm, int Nc, int32_t* ipiv, int32_t* status, size_t lds_limit, hipStream_t st);
+template <typename TX>
+hipError_t launch_last_solve(const double* Hm, const int32_t* ipiv, int N, int n, int qr, int qx, int nr, int nx,
+                             TX* x, int64_t ldx, int nrhs, size_t lds_limit, hipStream_t st);
+
+template <typename TA>
+hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st);
+hipError_t launch_nrm2(const double* v, int64_t len, double* tmp, int max_chunks, double* out, hipStream_t st);
+
+// dst[i] = TD(src[i]) for i < count  (also a zero-fill when src == nullptr)
+template <typename TS, typename TD>
+hipError_t launch_convert(const TS* src, TD* dst, int64_t count, hipStream_t st);
+// dst[i] += TD(alpha * src[i])
+hipError_t launch_axpy(double alpha, const double* src, double* dst, int64_t count, hipStream_t st);
+
+}  // namespace bcr

@@ -1,0 +1,131 @@
+// FP64 block SpMV of the bordered BABD operator (BABD_BorderedCR.cc:2111-2166):
+// r = b - A x (residual mode) or y = A x, on the pristine (unfactorized) blocks.
+//
+// mv_rows_kernel   one wave per block row k: the n rows D_k x_k + E_k x_{k+1}
+//                  + B_k xb, and the border partial C_k x_k -> Cpart[k]
+//                  (k = 0..N; row N contributes only its border partial)
+// mv_tail_kernel   the n+qr boundary rows and the nr border rows
+//                  (sum of the partials + Cq q + F xb)
+// The per-block partials make the border rows a deterministic reduction
+// instead of N serial gemv accumulations.
+#include "bcr_launch.hh"
+
+namespace bcr {
+
+
+template <typename TA>
+__global__ void __launch_bounds__(256) mv_rows_kernel(MvArgs<TA> a) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int N = a.N, n = a.n, nx = a.nx, nr = a.nr;
+  if (k > N) return;
+  const double* xk = a.x + size_t(k) * n;
+  const double* xb = a.x + size_t(N + 1) * n + a.qx;
+  if (k < N) {
+    const TA* Dk = a.D + size_t(k) * n * n;
+    const TA* Ek = a.E + size_t(k) * n * n;
+    const TA* Bk = a.B + size_t(k) * n * nx;
+    for (int r = lane; r < n; r += 64) {
+      double acc = 0;
+      for (int c = 0; c < n; ++c) acc += double(Dk[r + c * n]) * xk[c] + double(Ek[r + c * n]) * xk[n + c];
+      for (int c = 0; c < nx; ++c) acc += double(Bk[r + c * n]) * xb[c];
+      const size_t row = size_t(k) * n + r;
+      a.out[row] = a.b ? a.b[row] - acc : acc;
+    }
+  }
+  const TA* Ck = a.C + size_t(k) * nr * n;
+  for (int i = lane; i < nr; i += 64) {
+    double acc = 0;
+    for (int c = 0; c < n; ++c) acc += double(Ck[i + c * nr]) * xk[c];
+    a.Cpart[size_t(k) * nr + i] = acc;
+  }
+}
+
+template <typename TA>
+__global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double* __restrict__ Csum) {
+  const int N = a.N, n = a.n, qr = a.qr, qx = a.qx, nr = a.nr, nx = a.nx;
+  const int m = n + qr;
+  const int Nc = 2 * n + qx + nx;
+  const double* x0 = a.x;
+  const double* xN = a.x + size_t(N) * n;
+  const double* xq = a.x + size_t(N + 1) * n;
+  const double* xb = xq + qx;
+  for (int i = threadIdx.x; i < m + nr; i += blockDim.x) {
+    double acc = 0;
+    size_t row;
+    if (i < m) {
+      for (int c = 0; c < Nc; ++c) {
+        double xv;
+        if (c < n) xv = x0[c];
+        else if (c < 2 * n) xv = xN[c - n];
+        else if (c < 2 * n + qx) xv = xq[c - 2 * n];
+        else xv = xb[c - 2 * n - qx];
+        acc += double(a.H[i + size_t(c) * m]) * xv;
+      }
+      row = size_t(N) * n + i;
+    } else {
+      const int r = i - m;
+      acc = Csum[r];
+      for (int c = 0; c < qx; ++c) acc += double(a.Cq[r + c * nr]) * xq[c];
+      for (int c = 0; c < nx; ++c) acc += double(a.F[r + c * nr]) * xb[c];
+      row = size_t(N + 1) * n + qr + r;
+    }
+    a.out[row] = a.b ? a.b[row] - acc : acc;
+  }
+}
+
+// sum of squares of v[0..len) -> tmp[blockIdx] (pass 1) ; out = sqrt(sum) (pass 2)
+__global__ void __launch_bounds__(256) sumsq_partial_kernel(const double* __restrict__ v, int64_t len,
+                                                            double* __restrict__ tmp) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += int64_t(gridDim.x) * blockDim.x)
+    s += v[i] * v[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tmp[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) sumsq_final_kernel(const double* __restrict__ tmp, int nb,
+                                                          double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += tmp[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(red[0]);
+}
+
+template <typename TA>
+hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st) {
+  hipLaunchKernelGGL((mv_rows_kernel<TA>), dim3((a.N + 1 + 3) / 4), dim3(256), 0, st, a);
+  if (a.nr > 0) {
+    launch_convert<double, double>(nullptr, Csum, a.nr, st);
+    launch_reduce<double, double>(a.Cpart, 0, a.N + 1, a.nr, tmp, max_chunks, Csum, a.nr, a.nr, st);
+  }
+  hipLaunchKernelGGL((mv_tail_kernel<TA>), dim3(1), dim3(256), 0, st, a, Csum);
+  return hipGetLastError();
+}
+
+hipError_t launch_nrm2(const double* v, int64_t len, double* tmp, int max_chunks, double* out, hipStream_t st) {
+  int64_t nb64 = (len + 255) / 256;
+  if (nb64 < 1) nb64 = 1;
+  if (nb64 > max_chunks) nb64 = max_chunks;
+  const int nb = int(nb64);
+  hipLaunchKernelGGL(sumsq_partial_kernel, dim3(nb), dim3(256), 0, st, v, len, tmp);
+  hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(256), 0, st, tmp, nb, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_mv<double>(MvArgs<double> const&, double*, double*, int, hipStream_t);
+template hipError_t launch_mv<float>(MvArgs<float> const&, double*, double*, int, hipStream_t);
+
+}  // namespace bcr

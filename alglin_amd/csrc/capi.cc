@@ -1,0 +1,497 @@
+// extern "C" boundary of alglin_amd (declarations in include/alglin_amd.h).
+#include "../../include/alglin_amd.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "BorderedCR.hh"
+
+using alglin_amd::BorderedCR;
+
+struct alglin_bcr {
+  int dtype = 0;
+  std::unique_ptr<BorderedCR<double>> d;
+  std::unique_ptr<BorderedCR<float>> f;
+  std::string info, err;
+  bool bound = false;
+};
+
+namespace {
+
+template <typename Fn>
+int guarded(alglin_bcr_t h, Fn&& fn) {
+  if (!h) return -1;
+  try {
+    return fn();
+  } catch (std::exception const& e) {
+    h->err = e.what();
+    return 1;
+  }
+}
+
+template <typename Fn>
+auto on(alglin_bcr_t h, Fn&& fn) {
+  return h->dtype == 0 ? fn(*h->d, (double*)nullptr) : fn(*h->f, (float*)nullptr);
+}
+
+int on_(alglin_bcr_t h, int on) {
+  if (h->dtype == 0) h->d->engine().set_timing(on != 0);
+  else h->f->engine().set_timing(on != 0);
+  return 0;
+}
+
+template <typename S>
+int result(alglin_bcr_t h, S& s, bool ok) {
+  if (!ok) h->err = s.last_error();
+  return ok ? 0 : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+alglin_bcr_t alglin_bcr_create(int dtype) {
+  try {
+    auto* h = new alglin_bcr;
+    h->dtype = dtype == 1 ? 1 : 0;
+    if (h->dtype == 0) h->d.reset(new BorderedCR<double>(nullptr));
+    else h->f.reset(new BorderedCR<float>(nullptr));
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void alglin_bcr_destroy(alglin_bcr_t h) { delete h; }
+
+int alglin_bcr_allocate(alglin_bcr_t h, int32_t N, int32_t n, int32_t qr, int32_t qx, int32_t nr, int32_t nx) {
+  return guarded(h, [&] {
+    h->bound = false;
+    return on(h, [&](auto& s, auto*) { s.allocate(N, n, qr, qx, nr, nx); return 0; });
+  });
+}
+
+int alglin_bcr_select(alglin_bcr_t h, char const* internal, char const* last, char const* last2) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) {
+      if (internal) s.select(internal);
+      if (last) s.select_last(last);
+      if (last2) s.select_last2(last2);
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_dims(alglin_bcr_t h, int32_t out[10]) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) {
+      int32_t v[10] = {s.number_of_blocks(), s.block_size(), s.dim_qr(), s.dim_qx(), s.dim_nr(),
+                       s.dim_nx(), s.nrows(), s.ncols(), s.Nr(), s.Nc()};
+      std::memcpy(out, v, sizeof(v));
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_load(alglin_bcr_t h, char const* what, int32_t nbl, void const* A, int32_t ld) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      T const* p = static_cast<T const*>(A);
+      std::string const w(what ? what : "");
+      if (w == "B") s.load_B(nbl, p, ld);
+      else if (w == "C") s.load_C(nbl, p, ld);
+      else if (w == "D") s.load_D(nbl, p, ld);
+      else if (w == "E") s.load_E(nbl, p, ld);
+      else if (w == "DE") s.load_DE(nbl, p, ld);
+      else if (w == "DEB") s.load_DEB(nbl, p, ld);
+      else if (w == "F") s.load_F(p, ld);
+      else if (w == "Cq") s.load_Cq(p, ld);
+      else if (w == "CqF") s.load_CqF(p, ld);
+      else if (w == "H") s.load_bottom(p, ld);
+      else if (w == "add_B") s.add_to_B(nbl, p, ld);
+      else if (w == "add_C") s.add_to_C(nbl, p, ld);
+      else if (w == "add_C2") s.add_to_C2(nbl, p, ld);
+      else if (w == "add_C2F") s.add_to_C2F(nbl, p, ld);
+      else if (w == "add_F") s.add_to_F(p, ld);
+      else throw std::invalid_argument("alglin_bcr_load: unknown block '" + w + "'");
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_load_bottom(alglin_bcr_t h, void const* H0, int32_t ld0, void const* HN, int32_t ldN, void const* Hq,
+                           int32_t ldQ, void const* Hp, int32_t ldP) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      s.load_bottom(static_cast<T const*>(H0), ld0, static_cast<T const*>(HN), ldN, static_cast<T const*>(Hq), ldQ,
+                    static_cast<T const*>(Hp), ldP);
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_load_bottom2(alglin_bcr_t h, void const* C0, int32_t ld0, void const* CN, int32_t ldN, void const* Cq,
+                            int32_t ldCq, void const* F, int32_t ldF) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      s.load_bottom2(static_cast<T const*>(C0), ld0, static_cast<T const*>(CN), ldN, static_cast<T const*>(Cq), ldCq,
+                     static_cast<T const*>(F), ldF);
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_load_packed(alglin_bcr_t h, void const* D, void const* E, void const* B, void const* C, void const* H,
+                           void const* Cq, void const* F) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      const size_t N = s.number_of_blocks(), n = s.block_size(), nr = s.dim_nr(), nx = s.dim_nx();
+      auto cp = [](void const* src, T* dst, size_t cnt) { if (src && cnt) std::memcpy(dst, src, cnt * sizeof(T)); };
+      cp(D, s.data_D(), N * n * n);
+      cp(E, s.data_E(), N * n * n);
+      cp(B, s.data_B(), N * n * nx);
+      cp(C, s.data_C(), (N + 1) * nr * n);
+      cp(H, s.data_H(), (n + s.dim_qr()) * size_t(s.Nc()));
+      cp(Cq, s.data_Cq(), nr * s.dim_qx());
+      cp(F, s.data_F(), nr * nx);
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_factorize(alglin_bcr_t h) {
+  return guarded(h, [&] { return on(h, [&](auto& s, auto*) { return result(h, s, s.factorize()); }); });
+}
+
+int alglin_bcr_solve(alglin_bcr_t h, void* x) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      return result(h, s, s.solve(static_cast<T*>(x)));
+    });
+  });
+}
+
+int alglin_bcr_solve_n(alglin_bcr_t h, int32_t nrhs, void* X, int32_t ld) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      return result(h, s, s.solve(nrhs, static_cast<T*>(X), ld));
+    });
+  });
+}
+
+int alglin_bcr_mv(alglin_bcr_t h, void const* x, void* y) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      s.Mv(static_cast<T const*>(x), static_cast<T*>(y));
+      return 0;
+    });
+  });
+}
+
+int32_t alglin_bcr_sparse_nnz(alglin_bcr_t h) {
+  if (!h) return -1;
+  return on(h, [&](auto& s, auto*) { return s.sparse_nnz(); });
+}
+
+int alglin_bcr_sparse_pattern(alglin_bcr_t h, int32_t* I, int32_t* J, int32_t offs) {
+  return guarded(h, [&] { return on(h, [&](auto& s, auto*) { s.sparse_pattern(I, J, offs); return 0; }); });
+}
+
+int alglin_bcr_sparse_values(alglin_bcr_t h, void* V) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      s.sparse_values(static_cast<T*>(V));
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_sparse_load(alglin_bcr_t h, void const* V, int32_t const* I, int32_t r_offs, int32_t const* J,
+                           int32_t c_offs, int32_t nnz, int do_error) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      s.sparse_load(static_cast<T const*>(V), I, r_offs, J, c_offs, nnz, do_error != 0);
+      return do_error ? 0 : (s.last_error() == "no error" ? 0 : 0);
+    });
+  });
+}
+
+int alglin_bcr_check_matrix(alglin_bcr_t h) {
+  return guarded(h, [&] { return on(h, [&](auto& s, auto*) { s.check_matrix(); return 0; }); });
+}
+
+char const* alglin_bcr_last_error(alglin_bcr_t h) {
+  if (!h) return "null handle";
+  if (!h->err.empty()) return h->err.c_str();
+  return on(h, [&](auto& s, auto*) { return s.last_error().c_str(); });
+}
+
+char const* alglin_bcr_info(alglin_bcr_t h) {
+  if (!h) return "";
+  h->info = on(h, [&](auto& s, auto*) { return s.info(); });
+  return h->info.c_str();
+}
+
+int alglin_bcr_bind_device(alglin_bcr_t h, void const* D, void const* E, void const* B, void const* C, void const* H,
+                           void const* Cq, void const* F) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      s.engine().bind_device(static_cast<T const*>(D), static_cast<T const*>(E), static_cast<T const*>(B),
+                             static_cast<T const*>(C), static_cast<T const*>(H), static_cast<T const*>(Cq),
+                             static_cast<T const*>(F));
+      h->bound = true;
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_factorize_device(alglin_bcr_t h, int sync) {
+  return guarded(h, [&] {
+    if (!h->bound) throw std::runtime_error("alglin_bcr_factorize_device: no device operator bound");
+    return on(h, [&](auto& s, auto*) {
+      auto& e = s.engine();
+      e.set_method(int(s.selected()));
+      if (s.last_selected() != std::remove_reference_t<decltype(s)>::LastChoice::LU)
+        throw std::runtime_error("alglin_bcr_factorize_device: only the LU last block is available on the GPU");
+      e.factorize_async();
+      if (!sync) return 0;
+      const int32_t st = e.status_sync();
+      if (st) h->err = st & 1 ? "zero pivot in a cyclic-reduction elimination" : "singular last block";
+      return st ? 1 : 0;
+    });
+  });
+}
+
+int alglin_bcr_solve_device(alglin_bcr_t h, void* X, int32_t nrhs, int64_t ld, int sync) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      auto& e = s.engine();
+      e.template solve_async<T>(static_cast<T*>(X), nrhs, ld);
+      if (sync) bcr::hip_check(hipStreamSynchronize(e.stream()), "sync");
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_residual_device(alglin_bcr_t h, double const* x, double const* b, double* r, int sync) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) {
+      auto& e = s.engine();
+      e.residual_async(x, b, r);
+      if (sync) bcr::hip_check(hipStreamSynchronize(e.stream()), "sync");
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_sync(alglin_bcr_t h) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) {
+      bcr::hip_check(hipStreamSynchronize(s.engine().stream()), "sync");
+      return 0;
+    });
+  });
+}
+
+void* alglin_bcr_stream(alglin_bcr_t h) {
+  if (!h) return nullptr;
+  return on(h, [&](auto& s, auto*) { return static_cast<void*>(s.engine().stream()); });
+}
+
+int alglin_bcr_get_factors(alglin_bcr_t h, void* T_, int32_t* perm, void* D, void* E, void* B, void* C,
+                           double* last_block, double* pending_sumsq) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      auto& e = s.engine();
+      const size_t N = s.number_of_blocks(), n = s.block_size(), nr = s.dim_nr(), nx = s.dim_nx();
+      auto cp = [&](void* dst, void const* src, size_t bytes) {
+        if (dst && bytes) bcr::hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H");
+      };
+      cp(T_, e.factor_T(), N * bcr::tsize(int(n)) * sizeof(T));
+      cp(perm, e.factor_perm(), N * n * sizeof(int32_t));
+      cp(D, e.work_D(), N * n * n * sizeof(T));
+      cp(E, e.work_E(), N * n * n * sizeof(T));
+      cp(B, e.work_B(), N * n * nx * sizeof(T));
+      cp(C, e.work_C(), (N + 1) * nr * n * sizeof(T));
+      cp(last_block, e.last_block(), size_t(s.Nr()) * s.Nc() * sizeof(double));
+      if (pending_sumsq) {
+        std::vector<T> pend(e.pending_count());
+        cp(pend.data(), e.pending(), pend.size() * sizeof(T));
+        double ss = 0;
+        for (T v : pend) ss += double(v) * double(v);
+        *pending_sumsq = ss;
+      }
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_fast_path(alglin_bcr_t h) {
+  if (!h) return 0;
+  return on(h, [&](auto& s, auto*) { return s.engine().stats().fast_path; });
+}
+
+int alglin_bcr_set_timing(alglin_bcr_t h, int on) {
+  return guarded(h, [&] { return on_(h, on); });
+}
+
+int alglin_bcr_timing(alglin_bcr_t h, double out[8], int reset) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) { s.engine().timing(out, reset != 0); return 0; });
+  });
+}
+
+void* alglin_dev_malloc(int64_t bytes) {
+  void* p = nullptr;
+  if (bytes <= 0) bytes = 1;
+  if (hipMalloc(&p, size_t(bytes)) != hipSuccess) return nullptr;
+  return p;
+}
+
+int alglin_dev_free(void* p) { return p && hipFree(p) == hipSuccess ? 0 : 1; }
+
+int alglin_memcpy_htod(void* dst, void const* src, int64_t bytes) {
+  return hipMemcpy(dst, src, size_t(bytes), hipMemcpyHostToDevice) == hipSuccess ? 0 : 1;
+}
+
+int alglin_memcpy_dtoh(void* dst, void const* src, int64_t bytes) {
+  return hipMemcpy(dst, src, size_t(bytes), hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+}
+
+int alglin_memcpy_dtod(void* dst, void const* src, int64_t bytes) {
+  return hipMemcpy(dst, src, size_t(bytes), hipMemcpyDeviceToDevice) == hipSuccess ? 0 : 1;
+}
+
+int alglin_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }
+
+int alglin_device_info(char* name, int32_t len, int32_t* cus, int64_t* hbm_bytes) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1;
+  if (name && len > 0) {
+    std::strncpy(name, prop.gcnArchName, size_t(len) - 1);
+    name[len - 1] = 0;
+  }
+  if (cus) *cus = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = int64_t(prop.totalGlobalMem);
+  return 0;
+}
+
+// ---- reference C interface -----------------------------------------------------
+
+namespace {
+std::mutex g_mu;
+std::map<BABD_intType, std::unique_ptr<BorderedCR<double>>> g_solvers;
+std::string g_last_error = "no error";
+
+BorderedCR<double>& solver_for(BABD_intType id) {
+  auto& p = g_solvers[id];
+  if (!p) p.reset(new BorderedCR<double>(nullptr));
+  return *p;
+}
+
+void configure(BorderedCR<double>& s, BABD_intType mat_fact, BABD_intType last_block_fact) {
+  switch (mat_fact) {
+    case 0: case 2: s.select("LU"); break;
+    case 3: s.select("QR"); break;
+    case 4: s.select("QRP"); break;
+    default: throw std::invalid_argument("BABD_factorize: mat_fact must be 0 (CR+LU), 2, 3 or 4");
+  }
+  static char const* names[] = {"LU", "LUPQ", "QR", "QRP", "SVD", "LSS", "LSY", "PINV"};
+  if (last_block_fact < 0 || last_block_fact > 7) throw std::invalid_argument("BABD_factorize: bad last_block_fact");
+  s.select_last(names[last_block_fact]);
+}
+
+int finish(BorderedCR<double>& s) {
+  if (!s.factorize()) {
+    g_last_error = s.last_error();
+    return 1;
+  }
+  return 0;
+}
+}  // namespace
+
+int BABD_factorize(BABD_intType id, BABD_intType mat_fact, BABD_intType last_block_fact, BABD_intType nblock,
+                   BABD_intType n, BABD_intType qr, BABD_intType qx, BABD_realType const DE[], BABD_intType ldDE,
+                   BABD_realType const H0[], BABD_intType ldH0, BABD_realType const HN[], BABD_intType ldHN,
+                   BABD_realType const Hq[], BABD_intType ldHq) {
+  std::lock_guard<std::mutex> g(g_mu);
+  try {
+    auto& s = solver_for(id);
+    s.allocate(nblock, n, qr, qx, 0, 0);
+    configure(s, mat_fact, last_block_fact);
+    for (BABD_intType k = 0; k < nblock; ++k) s.load_DE(k, DE + size_t(2) * n * k * ldDE, ldDE);
+    s.load_bottom(H0, ldH0, HN, ldHN, Hq, ldHq, nullptr, 1);
+    return finish(s);
+  } catch (std::exception const& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+int BABD_factorize_bordered(BABD_intType id, BABD_intType mat_fact, BABD_intType last_block_fact, BABD_intType nblock,
+                            BABD_intType n, BABD_intType qr, BABD_intType nr, BABD_intType qx, BABD_intType nx,
+                            BABD_realType const DE[], BABD_intType ldDE, BABD_realType const H0[], BABD_intType ldH0,
+                            BABD_realType const HN[], BABD_intType ldHN, BABD_realType const Hq[], BABD_intType ldHq,
+                            BABD_realType const B[], BABD_intType ldB, BABD_realType const C[], BABD_intType ldC,
+                            BABD_realType const D[], BABD_intType ldD) {
+  std::lock_guard<std::mutex> g(g_mu);
+  try {
+    auto& s = solver_for(id);
+    s.allocate(nblock, n, qr, qx, nr, nx);
+    configure(s, mat_fact, last_block_fact);
+    for (BABD_intType k = 0; k < nblock; ++k) {
+      s.load_DE(k, DE + size_t(2) * n * k * ldDE, ldDE);
+      // B is the full right border column block: rows k*n.. of B_k, then Hp
+      s.load_B(k, B + size_t(k) * n, ldB);
+    }
+    s.load_bottom(H0, ldH0, HN, ldHN, Hq, ldHq, B + size_t(nblock) * n, ldB);
+    // C is the full bottom border row block: C_0..C_N then Cq; D is the corner F
+    for (BABD_intType k = 0; k <= nblock; ++k) s.load_C(k, C + size_t(k) * n * ldC, ldC);
+    s.load_Cq(C + size_t(nblock + 1) * n * ldC, ldC);
+    s.load_F(D, ldD);
+    return finish(s);
+  } catch (std::exception const& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+int BABD_solve(BABD_intType id, BABD_realType rhs_sol[]) {
+  std::lock_guard<std::mutex> g(g_mu);
+  auto it = g_solvers.find(id);
+  if (it == g_solvers.end()) {
+    g_last_error = "BABD_solve: unknown mat_id";
+    return 1;
+  }
+  if (!it->second->solve(rhs_sol)) {
+    g_last_error = it->second->last_error();
+    return 1;
+  }
+  return 0;
+}
+
+char const* BABD_get_last_error(void) { return g_last_error.c_str(); }
+
+int BABD_free(BABD_intType id) {
+  std::lock_guard<std::mutex> g(g_mu);
+  return g_solvers.erase(id) ? 0 : 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+/*
+ * alglin_amd -- C ABI of the MI355X bordered cyclic-reduction BABD solver.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The reference's C interface for BorderedCR (BABD_*).  Its header
+ *     (Alglin/BABD_C_interface.h) is absent from the reference snapshot; the
+ *     signatures below are the ones src_tests/test9-Cinterface.c calls:
+ *       BABD_factorize            test9-Cinterface.c:90-91
+ *       BABD_factorize_bordered   test9-Cinterface.c:171-174
+ *       BABD_solve                test9-Cinterface.c:97
+ *       BABD_get_last_error       test9-Cinterface.c:92
+ *     mat_fact: 0 or 2 = cyclic reduction + LU, 3 = + QR, 4 = + QRP (the
+ *     BABD_Choice values of src/Alglin/BABD.hxx:66-71; 1 = Diaz is not part of
+ *     this solver and is rejected).  last_block_fact: the
+ *     BORDERED_LAST_Choice values (BABD_BorderedCR.hxx:118-127), 0 = LU.
+ *     Return 0 on success.
+ *
+ *  2. A handle API mirroring alglin::BorderedCR<T>
+ *     (src/Alglin/BABD_BorderedCR.hxx:104-1176): create/allocate
+ *     (hxx:373-442), the block loaders (cc:237-793), select* (hxx:458-540),
+ *     factorize (cc:803-839), solve and multi-RHS solve (cc:1489-1605),
+ *     Mv (cc:2111-2166), sparse export/import (cc:2178-2482), last_error
+ *     (hxx:445), info (cc:208-235); plus device-resident variants whose data
+ *     pointers are HIP device pointers (inputs already in HBM).
+ *
+ * Element type of a handle: dtype 0 = double, 1 = float; every `void*` data
+ * argument points to that type.  Matrices are column-major with a leading
+ * dimension, as in the reference.  Handle calls return 0 on success and
+ * nonzero on failure (message in alglin_bcr_last_error).
+ */
+#ifndef ALGLIN_AMD_H
+#define ALGLIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t BABD_intType;
+typedef double BABD_realType;
+
+/* ---- reference C interface (test9-Cinterface.c) ---------------------------- */
+int BABD_factorize(BABD_intType mat_id, BABD_intType mat_fact, BABD_intType last_block_fact, BABD_intType nblock,
+                   BABD_intType n, BABD_intType qr, BABD_intType qx, BABD_realType const DE[], BABD_intType ldDE,
+                   BABD_realType const H0[], BABD_intType ldH0, BABD_realType const HN[], BABD_intType ldHN,
+                   BABD_realType const Hq[], BABD_intType ldHq);
+int BABD_factorize_bordered(BABD_intType mat_id, BABD_intType mat_fact, BABD_intType last_block_fact,
+                            BABD_intType nblock, BABD_intType n, BABD_intType qr, BABD_intType nr, BABD_intType qx,
+                            BABD_intType nx, BABD_realType const DE[], BABD_intType ldDE, BABD_realType const H0[],
+                            BABD_intType ldH0, BABD_realType const HN[], BABD_intType ldHN, BABD_realType const Hq[],
+                            BABD_intType ldHq, BABD_realType const B[], BABD_intType ldB, BABD_realType const C[],
+                            BABD_intType ldC, BABD_realType const D[], BABD_intType ldD);
+int BABD_solve(BABD_intType mat_id, BABD_realType rhs_sol[]);
+char const* BABD_get_last_error(void);
+/* releases the GPU memory of mat_id (no counterpart in test9; callers that
+   reuse many ids need it) */
+int BABD_free(BABD_intType mat_id);
+
+/* ---- handle API (BorderedCR<T> surface) ------------------------------------ */
+typedef struct alglin_bcr* alglin_bcr_t;
+
+alglin_bcr_t alglin_bcr_create(int dtype);                                   /* hxx:373-384 */
+void alglin_bcr_destroy(alglin_bcr_t h);
+int alglin_bcr_allocate(alglin_bcr_t h, int32_t nblock, int32_t n, int32_t qr, int32_t qx, int32_t nr,
+                        int32_t nx);                                         /* cc:42-170 */
+int alglin_bcr_select(alglin_bcr_t h, char const* internal, char const* last, char const* last2); /* hxx:458-540 */
+int alglin_bcr_dims(alglin_bcr_t h, int32_t out[10]);  /* N n qr qx nr nx nrows ncols Nr Nc (hxx:617-650) */
+
+/* loaders (cc:237-793): `what` is one of "B","C","D","E","DE","DEB" (block nbl),
+   "F","Cq","CqF","H" (nbl ignored), "add_B","add_C","add_C2","add_C2F","add_F" */
+int alglin_bcr_load(alglin_bcr_t h, char const* what, int32_t nbl, void const* A, int32_t ld);
+int alglin_bcr_load_bottom(alglin_bcr_t h, void const* H0, int32_t ld0, void const* HN, int32_t ldN, void const* Hq,
+                           int32_t ldQ, void const* Hp, int32_t ldP);   /* cc:289-305 */
+int alglin_bcr_load_bottom2(alglin_bcr_t h, void const* C0, int32_t ld0, void const* CN, int32_t ldN, void const* Cq,
+                            int32_t ldCq, void const* F, int32_t ldF); /* cc:380-395 */
+/* all blocks at once, packed in the reference's internal layout (hxx:972-998):
+   D,E: N n*n blocks; B: N n*nx; C: (N+1) nr*n; H: (n+qr) x Nc; Cq: nr x qx; F: nr x nx */
+int alglin_bcr_load_packed(alglin_bcr_t h, void const* D, void const* E, void const* B, void const* C,
+                           void const* H, void const* Cq, void const* F);
+
+int alglin_bcr_factorize(alglin_bcr_t h);                                    /* cc:803-839 */
+int alglin_bcr_solve(alglin_bcr_t h, void* x);                               /* cc:1489-1542 */
+int alglin_bcr_solve_n(alglin_bcr_t h, int32_t nrhs, void* X, int32_t ld);   /* cc:1546-1605 */
+int alglin_bcr_mv(alglin_bcr_t h, void const* x, void* y);                   /* cc:2111-2116 */
+int32_t alglin_bcr_sparse_nnz(alglin_bcr_t h);                               /* hxx:1079-1088 */
+int alglin_bcr_sparse_pattern(alglin_bcr_t h, int32_t* I, int32_t* J, int32_t offs); /* cc:2342-2378 */
+int alglin_bcr_sparse_values(alglin_bcr_t h, void* V);                       /* cc:2382-2413 */
+int alglin_bcr_sparse_load(alglin_bcr_t h, void const* V, int32_t const* I, int32_t r_offs, int32_t const* J,
+                           int32_t c_offs, int32_t nnz, int do_error);       /* cc:2417-2482 */
+int alglin_bcr_check_matrix(alglin_bcr_t h);                                 /* cc:2538-2659 */
+char const* alglin_bcr_last_error(alglin_bcr_t h);                           /* hxx:445 */
+char const* alglin_bcr_info(alglin_bcr_t h);                                 /* cc:208-235 */
+
+/* ---- device-resident variants (HIP device pointers) ------------------------ */
+/* use caller-owned device arrays (packed layout as above) as the operator */
+int alglin_bcr_bind_device(alglin_bcr_t h, void const* D, void const* E, void const* B, void const* C,
+                           void const* H, void const* Cq, void const* F);
+/* enqueue the factorization of the bound operator; sync=1 waits and returns status */
+int alglin_bcr_factorize_device(alglin_bcr_t h, int sync);
+/* in-place solve of a device block X (ld x nrhs) */
+int alglin_bcr_solve_device(alglin_bcr_t h, void* X, int32_t nrhs, int64_t ld, int sync);
+/* FP64 residual r = b - A x on the bound operator (double device vectors) */
+int alglin_bcr_residual_device(alglin_bcr_t h, double const* x, double const* b, double* r, int sync);
+int alglin_bcr_sync(alglin_bcr_t h);
+/* the HIP stream the handle launches on (hipStream_t) */
+void* alglin_bcr_stream(alglin_bcr_t h);
+/* factor arrays (host copies) for tests: T panels, Perm, D, E, B, C after
+   factorize; any pointer may be null */
+int alglin_bcr_get_factors(alglin_bcr_t h, void* T, int32_t* perm, void* D, void* E, void* B, void* C,
+                           double* last_block, double* pending_sumsq);
+/* 1 when the last factorization used the register-resident LU kernel */
+int alglin_bcr_fast_path(alglin_bcr_t h);
+
+/* ---- per-phase GPU timing (HIP events on the handle's stream) --------------- */
+/* on=1 records an event pair around every launch group from now on */
+int alglin_bcr_set_timing(alglin_bcr_t h, int on);
+/* accumulated milliseconds since the last reset and launch counts:
+   out[0] pair eliminations  out[1] forward sweeps  out[2] backward sweeps
+   out[3] last block (assemble+LU+solve)  out[4] reductions/copies
+   out[5] residual SpMV  out[6] launches of pair-elimination kernels
+   out[7] launches of sweep kernels; reset=1 clears after reading */
+int alglin_bcr_timing(alglin_bcr_t h, double out[8], int reset);
+
+/* ---- device memory (no other runtime needed) -------------------------------- */
+void* alglin_dev_malloc(int64_t bytes);
+int alglin_dev_free(void* p);
+int alglin_memcpy_htod(void* dst, void const* src, int64_t bytes);
+int alglin_memcpy_dtoh(void* dst, void const* src, int64_t bytes);
+int alglin_memcpy_dtod(void* dst, void const* src, int64_t bytes);
+int alglin_device_synchronize(void);
+/* device name, compute units, HBM bytes */
+int alglin_device_info(char* name, int32_t len, int32_t* cus, int64_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ALGLIN_AMD_H */
