@@ -60,6 +60,8 @@ def lib():
             "alglin_bcr_get_factors": (ci, [vp] * 9),
             "alglin_bcr_fast_path": (ci, [vp]),
             "alglin_bcr_set_timing": (ci, [vp, ci]),
+            "alglin_bcr_set_precision": (ci, [vp, cs, ctypes.c_double, i32]),
+            "alglin_bcr_refine_report": (ci, [vp, vp]),
             "alglin_bcr_timing": (ci, [vp, vp, ci]),
             "alglin_dev_malloc": (vp, [i64]),
             "alglin_dev_free": (ci, [vp]),
@@ -232,6 +234,16 @@ class BorderedCR:
         self._bound = t
         ptr = [ctypes.c_void_p(t[k].data_ptr()) for k in ("D", "E", "B", "C", "H", "Cq", "F")]
         self._check(lib().alglin_bcr_bind_device(self._h, *ptr), "bind_device")
+
+    def select_precision(self, mode="FP64", tol=1e-12, max_iter=30):
+        """'FP64' (reference arithmetic) or 'MIXED' (FP32 factors + FP64 refinement)."""
+        self._check(lib().alglin_bcr_set_precision(self._h, mode.encode(), tol, max_iter), "select_precision")
+
+    def refine_report(self) -> dict:
+        out = np.zeros(4)
+        self._check(lib().alglin_bcr_refine_report(self._h, _p(out)), "refine_report")
+        return dict(iterations=int(out[0]), rel_residual=float(out[1]), first_residual=float(out[2]),
+                    status=int(out[3]))
 
     def set_timing(self, on=True):
         self._check(lib().alglin_bcr_set_timing(self._h, int(on)), "set_timing")

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <sstream>
 #include <stdexcept>
+#include <type_traits>
 
 namespace alglin_amd {
 
@@ -61,6 +62,7 @@ void BorderedCR<T>::allocate(integer N, integer n, integer qr, integer qx, integ
   H_.assign(size_t(n + qr) * d.Nc(), T(0));
   try {
     eng_->allocate(d_);
+    if (mix_) mix_->allocate(d_);
   } catch (std::exception const& e) {
     last_error_ = e.what();
     throw;
@@ -225,10 +227,21 @@ bool BorderedCR<T>::factorize() {
       last_error_ = std::string("BorderedCR::factorize, ") + kLast[int(last_)] + " is not available on the GPU path";
       return false;
     }
-    eng_->set_method(int(sel_));
-    eng_->upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
-    eng_->factorize_async();
-    int32_t const st = eng_->status_sync();
+    int32_t st = 0;
+    if (mixed_) {
+      if constexpr (std::is_same_v<T, double>) {
+        auto& e = mix_->engine();
+        e.set_method(int(sel_));
+        e.upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
+        e.factorize_async();
+        st = e.status_sync();
+      }
+    } else {
+      eng_->set_method(int(sel_));
+      eng_->upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
+      eng_->factorize_async();
+      st = eng_->status_sync();
+    }
     if (st & 1) {
       last_error_ = "BorderedCR::buildT, zero pivot in a cyclic-reduction elimination";
       return false;
@@ -260,17 +273,92 @@ bool BorderedCR<T>::solve(integer nrhs, T X[], integer ld) {
     const integer L = std::max(nrows(), ncols());
     if (ld < L) throw std::invalid_argument("BorderedCR::solve, ldRhs < max(nrows,ncols)");
     xbuf_.resize(size_t(L) * nrhs);
+    hipStream_t st = eng_->stream();
+    if constexpr (std::is_same_v<T, double>)
+      if (mixed_) st = mix_->engine().stream();
     bcr::hip_check(hipMemcpy2DAsync(xbuf_.p, size_t(L) * sizeof(T), X, size_t(ld) * sizeof(T), size_t(L) * sizeof(T),
-                                    nrhs, hipMemcpyHostToDevice, eng_->stream()), "H2D x");
-    eng_->template solve_async<T>(xbuf_.p, nrhs, L);
+                                    nrhs, hipMemcpyHostToDevice, st), "H2D x");
+    solve_device(xbuf_.p, nrhs, L, false);
     bcr::hip_check(hipMemcpy2DAsync(X, size_t(ld) * sizeof(T), xbuf_.p, size_t(L) * sizeof(T), size_t(L) * sizeof(T),
-                                    nrhs, hipMemcpyDeviceToHost, eng_->stream()), "D2H x");
-    bcr::hip_check(hipStreamSynchronize(eng_->stream()), "solve sync");
+                                    nrhs, hipMemcpyDeviceToHost, st), "D2H x");
+    bcr::hip_check(hipStreamSynchronize(st), "solve sync");
+    if (mixed_ && report_.status != 0) {
+      last_error_ = "BorderedCR::solve, iterative refinement did not reach the requested residual";
+      return false;
+    }
   } catch (std::exception const& e) {
     last_error_ = e.what();
     return false;
   }
   return true;
+}
+
+template <typename T>
+void BorderedCR<T>::select_precision(std::string_view mode, double tol, int max_iter) {
+  if (mode == "FP64" || mode == "NATIVE") {
+    mixed_ = false;
+  } else if (mode == "MIXED") {
+    if constexpr (!std::is_same_v<T, double>)
+      throw std::invalid_argument("BorderedCR::select_precision: MIXED needs the double solver");
+    if (!mix_) {
+      mix_.reset(new bcr::MixedEngine());
+      if (d_.N > 0) mix_->allocate(d_);
+    }
+    mixed_ = true;
+  } else {
+    throw std::invalid_argument("BorderedCR::select_precision( mode='" + std::string(mode) +
+                                "' ) unknown mode ('FP64','MIXED')");
+  }
+  ir_tol_ = tol;
+  ir_max_ = max_iter;
+  factorized_ = false;
+}
+
+template <typename T>
+void BorderedCR<T>::bind_device(T const* D, T const* E, T const* B, T const* C, T const* H, T const* Cq, T const* F) {
+  eng_->bind_device(D, E, B, C, H, Cq, F);
+  if constexpr (std::is_same_v<T, double>)
+    if (mix_) mix_->engine().bind_device(D, E, B, C, H, Cq, F);
+}
+
+template <typename T>
+int32_t BorderedCR<T>::factorize_device(bool sync) {
+  if (last_ != LastChoice::LU) throw std::runtime_error("factorize_device: only the LU last block is available on the GPU");
+  int32_t st = 0;
+  if (mixed_) {
+    if constexpr (std::is_same_v<T, double>) {
+      auto& e = mix_->engine();
+      e.set_method(int(sel_));
+      e.factorize_async();
+      if (sync) st = e.status_sync();
+    }
+  } else {
+    eng_->set_method(int(sel_));
+    eng_->factorize_async();
+    if (sync) st = eng_->status_sync();
+  }
+  factorized_ = st == 0;
+  return st;
+}
+
+// in-place solve of device columns; in MIXED mode each column is refined
+template <typename T>
+void BorderedCR<T>::solve_device(T* X, integer nrhs, int64_t ld, bool sync) {
+  if (mixed_) {
+    if constexpr (std::is_same_v<T, double>) {
+      report_ = bcr::RefineReport{};
+      for (integer c = 0; c < nrhs; ++c) {
+        bcr::RefineReport r = mix_->solve(X + size_t(c) * ld, ir_tol_, ir_max_);
+        report_.iterations = std::max(report_.iterations, r.iterations);
+        report_.rel_residual = std::max(report_.rel_residual, r.rel_residual);
+        report_.first_residual = std::max(report_.first_residual, r.first_residual);
+        report_.status = std::max(report_.status, r.status);
+      }
+    }
+    return;
+  }
+  eng_->template solve_async<T>(X, nrhs, ld);
+  if (sync) bcr::hip_check(hipStreamSynchronize(eng_->stream()), "solve sync");
 }
 
 template <typename T>

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "bcr_engine.hh"
+#include "bcr_refine.hh"
 
 namespace alglin_amd {
 
@@ -140,6 +141,18 @@ class BorderedCR {
 
   bcr::Engine<T, T>& engine() { return *eng_; }
 
+  // ---- precision (extension; the reference surface is the "FP64" default) --
+  // "MIXED" (double only): FP32 cyclic-reduction factors + FP64 iterative
+  // refinement on the GPU to ||b - A x|| / ||b|| <= tol.
+  void select_precision(std::string_view mode, double tol = 1e-12, int max_iter = 30);
+  bool mixed() const { return mixed_; }
+  bcr::RefineReport const& refine_report() const { return report_; }
+  bcr::MixedEngine* mixed_engine() { return mix_.get(); }
+  // device-resident entry points honouring the precision mode
+  void bind_device(T const* D, T const* E, T const* B, T const* C, T const* H, T const* Cq, T const* F);
+  int32_t factorize_device(bool sync);
+  void solve_device(T* X, integer nrhs, int64_t ld, bool sync);
+
  private:
   void fail(std::string const& msg) const;
   bcr::Dims d_;
@@ -149,6 +162,11 @@ class BorderedCR {
   bool fac_thread_ = true, sol_thread_ = true, debug_ = false, factorized_ = false;
   std::vector<T> D_, E_, B_, C_, Cq_, F_, H_;
   std::unique_ptr<bcr::Engine<T, T>> eng_;
+  std::unique_ptr<bcr::MixedEngine> mix_;
+  bool mixed_ = false;
+  double ir_tol_ = 1e-12;
+  int ir_max_ = 30;
+  bcr::RefineReport report_;
   bcr::DevBuf<T> xbuf_;
   mutable std::string last_error_{"no error"};
 };

@@ -1,0 +1,38 @@
+// Mixed-precision solver: FP32 cyclic-reduction factors of the FP64 operator
+// plus FP64 iterative refinement on the GPU.
+//
+//   x0 = M^{-1} b                      (M = FP32 factors, sweeps carried in FP64)
+//   r_k = b - A x_k                    (FP64 block SpMV on the pristine A)
+//   x_{k+1} = x_k + M^{-1} r_k
+//
+// until ||r_k||_2 / ||b||_2 <= tol, or three sweeps in a row fail to bring
+// the best residual so far down by a factor `stall` (status = stalled: the
+// FP64 noise floor of the operator is above tol), or max_iter.
+// All vectors stay in HBM; the only host traffic per sweep is the two norms.
+#pragma once
+
+#include "bcr_engine.hh"
+
+namespace bcr {
+
+struct RefineReport {
+  int iterations = 0;        // correction sweeps after the first solve
+  double rel_residual = 0;   // ||b - A x|| / ||b|| at exit (FP64)
+  double first_residual = 0; // after the first (FP32-factor) solve
+  int status = 0;            // 0 converged, 1 stalled, 2 max_iter, 3 factorization failed
+};
+
+class MixedEngine {
+ public:
+  MixedEngine() = default;
+  Engine<double, float>& engine() { return eng_; }
+  void allocate(Dims const& d);
+  // x: device vector, b on entry (length max(nrows,ncols)), solution on exit
+  RefineReport solve(double* x, double tol, int max_iter, double stall = 0.5);
+
+ private:
+  Engine<double, float> eng_;
+  DevBuf<double> b_, r_, norms_;
+};
+
+}  // namespace bcr

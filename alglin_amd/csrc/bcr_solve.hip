@@ -189,14 +189,18 @@ __global__ void __launch_bounds__(256) reduce_partial_kernel(const TP* __restric
   }
 }
 
+// one wave per output element: lanes stride over the chunk partials, then a
+// wave sum (deterministic order)
 template <typename TP, typename TD>
-__global__ void __launch_bounds__(256) reduce_final_kernel(const TP* __restrict__ tmp, int nchunks, int m,
-                                                           TD* __restrict__ dst, int mrow, int64_t ldd) {
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    TP s = 0;
-    for (int b = 0; b < nchunks; ++b) s += tmp[size_t(b) * m + i];
-    dst[(i % mrow) + size_t(i / mrow) * ldd] += TD(s);
-  }
+__global__ void __launch_bounds__(64) reduce_final_kernel(const TP* __restrict__ tmp, int nchunks, int m,
+                                                          TD* __restrict__ dst, int mrow, int64_t ldd) {
+  const int i = blockIdx.x;
+  if (i >= m) return;
+  TP s = 0;
+  for (int b = threadIdx.x; b < nchunks; b += 64) s += tmp[size_t(b) * m + i];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+  if (threadIdx.x == 0) dst[(i % mrow) + size_t(i / mrow) * ldd] += TD(s);
 }
 
 template <typename TW, typename TX>
@@ -227,7 +231,7 @@ hipError_t launch_reduce(const TP* part, int first, int count, int m, TP* tmp, i
   int nchunks = min(max_chunks, max(1, count / 16));
   hipLaunchKernelGGL((reduce_partial_kernel<TP, TD>), dim3(nchunks), dim3(256), 0, st, part, first, count, m, tmp,
                      nchunks);
-  hipLaunchKernelGGL((reduce_final_kernel<TP, TD>), dim3(1), dim3(256), 0, st, tmp, nchunks, m, dst, mrow, ldd);
+  hipLaunchKernelGGL((reduce_final_kernel<TP, TD>), dim3(m), dim3(64), 0, st, tmp, nchunks, m, dst, mrow, ldd);
   return hipGetLastError();
 }
 

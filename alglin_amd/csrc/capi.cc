@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "BorderedCR.hh"
 
@@ -39,8 +40,12 @@ auto on(alglin_bcr_t h, Fn&& fn) {
 }
 
 int on_(alglin_bcr_t h, int on) {
-  if (h->dtype == 0) h->d->engine().set_timing(on != 0);
-  else h->f->engine().set_timing(on != 0);
+  if (h->dtype == 0) {
+    h->d->engine().set_timing(on != 0);
+    if (h->d->mixed_engine()) h->d->mixed_engine()->engine().set_timing(on != 0);
+  } else {
+    h->f->engine().set_timing(on != 0);
+  }
   return 0;
 }
 
@@ -250,9 +255,9 @@ int alglin_bcr_bind_device(alglin_bcr_t h, void const* D, void const* E, void co
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto* tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      s.engine().bind_device(static_cast<T const*>(D), static_cast<T const*>(E), static_cast<T const*>(B),
-                             static_cast<T const*>(C), static_cast<T const*>(H), static_cast<T const*>(Cq),
-                             static_cast<T const*>(F));
+      s.bind_device(static_cast<T const*>(D), static_cast<T const*>(E), static_cast<T const*>(B),
+                    static_cast<T const*>(C), static_cast<T const*>(H), static_cast<T const*>(Cq),
+                    static_cast<T const*>(F));
       h->bound = true;
       return 0;
     });
@@ -263,13 +268,7 @@ int alglin_bcr_factorize_device(alglin_bcr_t h, int sync) {
   return guarded(h, [&] {
     if (!h->bound) throw std::runtime_error("alglin_bcr_factorize_device: no device operator bound");
     return on(h, [&](auto& s, auto*) {
-      auto& e = s.engine();
-      e.set_method(int(s.selected()));
-      if (s.last_selected() != std::remove_reference_t<decltype(s)>::LastChoice::LU)
-        throw std::runtime_error("alglin_bcr_factorize_device: only the LU last block is available on the GPU");
-      e.factorize_async();
-      if (!sync) return 0;
-      const int32_t st = e.status_sync();
+      const int32_t st = s.factorize_device(sync != 0);
       if (st) h->err = st & 1 ? "zero pivot in a cyclic-reduction elimination" : "singular last block";
       return st ? 1 : 0;
     });
@@ -280,9 +279,27 @@ int alglin_bcr_solve_device(alglin_bcr_t h, void* X, int32_t nrhs, int64_t ld, i
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto* tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      auto& e = s.engine();
-      e.template solve_async<T>(static_cast<T*>(X), nrhs, ld);
-      if (sync) bcr::hip_check(hipStreamSynchronize(e.stream()), "sync");
+      s.solve_device(static_cast<T*>(X), nrhs, ld, sync != 0);
+      if (s.mixed() && s.refine_report().status != 0) {
+        h->err = "iterative refinement did not reach the requested residual";
+        return 1;
+      }
+      return 0;
+    });
+  });
+}
+
+int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32_t max_iter) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) { s.select_precision(mode ? mode : "FP64", tol, max_iter); return 0; });
+  });
+}
+
+int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) {
+      auto const& r = s.refine_report();
+      out[0] = r.iterations; out[1] = r.rel_residual; out[2] = r.first_residual; out[3] = r.status;
       return 0;
     });
   });
@@ -353,7 +370,17 @@ int alglin_bcr_set_timing(alglin_bcr_t h, int on) {
 
 int alglin_bcr_timing(alglin_bcr_t h, double out[8], int reset) {
   return guarded(h, [&] {
-    return on(h, [&](auto& s, auto*) { s.engine().timing(out, reset != 0); return 0; });
+    return on(h, [&](auto& s, auto*) {
+      s.engine().timing(out, reset != 0);
+      if constexpr (std::is_same_v<std::remove_reference_t<decltype(s)>, BorderedCR<double>>) {
+        if (s.mixed_engine()) {   // phases of the mixed engine add to those of the FP64 one
+          double m[8];
+          s.mixed_engine()->engine().timing(m, reset != 0);
+          for (int i = 0; i < 8; ++i) out[i] += m[i];
+        }
+      }
+      return 0;
+    });
   });
 }
 

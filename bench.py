@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-N", type=int, default=25000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["fp64", "mixed"], default="fp64",
+                    help="fp64: plain FP64 cyclic reduction; mixed: FP32 factors + FP64 refinement to 1e-12")
+    ap.add_argument("--kappa", type=float, default=0.0,
+                    help="if > 0, D_k = Q diag(s) Q^T with cond(D_k) log-uniform in [1e2, kappa]")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -114,7 +118,10 @@ def main():
 
     N, n, qr, qx, nr, nx = args.N, args.n, args.qr, args.qr, args.nr, args.nr
     t_gen = time.perf_counter()
-    s = systems.gaussian_system(N, n, qr, qx, nr, nx, shift=args.shift, seed=args.seed)
+    if args.kappa > 0:
+        s = systems.conditioned_system(N, n, qr, qx, nr, nx, kmin=1e2, kmax=args.kappa, seed=args.seed)
+    else:
+        s = systems.gaussian_system(N, n, qr, qx, nr, nx, shift=args.shift, seed=args.seed)
     nrows, ncols = systems.dims(s)
     xref = np.ones(ncols)
     b = systems.mv(s, xref)
@@ -126,6 +133,8 @@ def main():
     g = BorderedCR()
     g.allocate(N, n, qr, qx, nr, nx)
     g.select("LU", "LU", "NONE")
+    if args.mode == "mixed":
+        g.select_precision("MIXED", 1e-12, 30)
     g.bind_device(dev)
     L = max(nrows, ncols)
     bd = DeviceArray.from_host(np.concatenate([b, np.zeros(L - nrows)]))
@@ -161,6 +170,7 @@ def main():
     r = DeviceArray(nrows)
     g.residual_device(X, bd, r)
     rel_res = float(np.linalg.norm(r.to_host()) / np.linalg.norm(b))
+    refine = g.refine_report() if args.mode == "mixed" else None
     fwd_err = float(np.abs(x - xref).max())
     print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, rel. residual {rel_res:.2e}, "
           f"max fwd err {fwd_err:.2e}", file=sys.stderr, flush=True)
@@ -201,15 +211,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.mode == "fp64" else "f32 factors + f64 refinement",
         "data": "synthetic (seeded Gaussian blocks, b = A*ones)",
         "config": {
-            "workload": f"plain FP64 BorderedCR, large single system N={N} n={n} qr=qx={qr} nr=nx={nr}, "
-                        f"1 RHS, factorize+solve per step",
+            "workload": (f"{'plain FP64' if args.mode == 'fp64' else 'mixed FP32+FP64-IR'} BorderedCR, large "
+                         f"single system N={N} n={n} qr=qx={qr} nr=nx={nr}, 1 RHS, factorize+solve per step"),
             "N": N, "n": n, "qr": qr, "qx": qx, "nr": nr, "nx": nx, "nrhs": 1,
             "internal": "LU", "last_block": "LU", "parallelism": "single-gpu",
         },
         "rel_residual": rel_res,
+        "refinement": refine,
         "max_fwd_err": fwd_err,
         "ms_factor_solve_16rhs": ms16,
         "phase_ms_per_step": {k: v / args.steps for k, v in tm.items() if k.endswith("_ms")},

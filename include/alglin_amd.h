@@ -113,6 +113,15 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T, int32_t* perm, void* D, void
 /* 1 when the last factorization used the register-resident LU kernel */
 int alglin_bcr_fast_path(alglin_bcr_t h);
 
+/* ---- precision (extension of the reference surface) -------------------------- */
+/* mode "FP64" (default, the reference's arithmetic) or "MIXED" (double handles
+   only): FP32 cyclic-reduction factors + FP64 iterative refinement on the GPU
+   until ||b - A x||_2 / ||b||_2 <= tol or max_iter correction sweeps */
+int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32_t max_iter);
+/* last refined solve: out = {sweeps, rel. residual, residual after the first
+   solve, status (0 converged, 1 stalled, 2 max_iter)} (worst over columns) */
+int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]);
+
 /* ---- per-phase GPU timing (HIP events on the handle's stream) --------------- */
 /* on=1 records an event pair around every launch group from now on */
 int alglin_bcr_set_timing(alglin_bcr_t h, int on);
