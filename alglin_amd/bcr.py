@@ -62,12 +62,14 @@ def lib():
             "alglin_bcr_set_timing": (ci, [vp, ci]),
             "alglin_bcr_set_precision": (ci, [vp, cs, ctypes.c_double, i32]),
             "alglin_bcr_refine_report": (ci, [vp, vp]),
+            "alglin_bcr_set_fallback": (ci, [vp, ctypes.c_int]),
             "alglin_bcr_timing": (ci, [vp, vp, ci]),
             "alglin_dev_malloc": (vp, [i64]),
             "alglin_dev_free": (ci, [vp]),
             "alglin_memcpy_htod": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtoh": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtod": (ci, [vp, vp, i64]),
+            "alglin_memcpy_dtod_async": (ci, [vp, vp, i64, vp]),
             "alglin_device_synchronize": (ci, []),
             "alglin_device_info": (ci, [ctypes.c_char_p, i32, vp, vp]),
             "BABD_factorize": (ci, [i32] * 7 + [vp, i32] * 4),
@@ -239,6 +241,9 @@ class BorderedCR:
         """'FP64' (reference arithmetic) or 'MIXED' (FP32 factors + FP64 refinement)."""
         self._check(lib().alglin_bcr_set_precision(self._h, mode.encode(), tol, max_iter), "select_precision")
 
+    def set_fallback(self, on: bool):
+        self._check(lib().alglin_bcr_set_fallback(self._h, int(on)), "set_fallback")
+
     def refine_report(self) -> dict:
         out = np.zeros(4)
         self._check(lib().alglin_bcr_refine_report(self._h, _p(out)), "refine_report")
@@ -302,9 +307,16 @@ class DeviceArray:
             raise AlglinError("D2H copy failed")
         return out
 
-    def copy_from(self, other: "DeviceArray"):
-        if lib().alglin_memcpy_dtod(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr),
-                                     min(self.nbytes, other.nbytes)) != 0:
+    def copy_from(self, other: "DeviceArray", stream: int | None = None):
+        """Device copy; with `stream` (a solver's stream()) it is queued there
+        and ordered with that solver's work, else it completes before return."""
+        nb = min(self.nbytes, other.nbytes)
+        if stream is None:
+            rc = lib().alglin_memcpy_dtod(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr), nb)
+        else:
+            rc = lib().alglin_memcpy_dtod_async(ctypes.c_void_p(self.ptr), ctypes.c_void_p(other.ptr), nb,
+                                                ctypes.c_void_p(stream))
+        if rc != 0:
             raise AlglinError("D2D copy failed")
 
     def free(self):

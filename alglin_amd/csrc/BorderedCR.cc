@@ -223,21 +223,24 @@ bool BorderedCR<T>::factorize() {
   factorized_ = false;
   try {
     if (debug_) check_matrix();
-    if (last_ != LastChoice::LU) {
+    if (int(last_) > int(LastChoice::QRP)) {
       last_error_ = std::string("BorderedCR::factorize, ") + kLast[int(last_)] + " is not available on the GPU path";
       return false;
     }
     int32_t st = 0;
+    device_bound_ = false;   // the host blocks become the operator
     if (mixed_) {
       if constexpr (std::is_same_v<T, double>) {
         auto& e = mix_->engine();
         e.set_method(int(sel_));
+        e.set_last_method(int(last_));
         e.upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
         e.factorize_async();
         st = e.status_sync();
       }
     } else {
       eng_->set_method(int(sel_));
+      eng_->set_last_method(int(last_));
       eng_->upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
       eng_->factorize_async();
       st = eng_->status_sync();
@@ -282,7 +285,7 @@ bool BorderedCR<T>::solve(integer nrhs, T X[], integer ld) {
     bcr::hip_check(hipMemcpy2DAsync(X, size_t(ld) * sizeof(T), xbuf_.p, size_t(L) * sizeof(T), size_t(L) * sizeof(T),
                                     nrhs, hipMemcpyDeviceToHost, st), "D2H x");
     bcr::hip_check(hipStreamSynchronize(st), "solve sync");
-    if (mixed_ && report_.status != 0) {
+    if (mixed_ && report_.status != 0 && report_.status != 4) {
       last_error_ = "BorderedCR::solve, iterative refinement did not reach the requested residual";
       return false;
     }
@@ -302,6 +305,7 @@ void BorderedCR<T>::select_precision(std::string_view mode, double tol, int max_
       throw std::invalid_argument("BorderedCR::select_precision: MIXED needs the double solver");
     if (!mix_) {
       mix_.reset(new bcr::MixedEngine());
+      mix_->engine().share_stream(eng_->stream());   // one stream per object
       if (d_.N > 0) mix_->allocate(d_);
     }
     mixed_ = true;
@@ -316,6 +320,7 @@ void BorderedCR<T>::select_precision(std::string_view mode, double tol, int max_
 
 template <typename T>
 void BorderedCR<T>::bind_device(T const* D, T const* E, T const* B, T const* C, T const* H, T const* Cq, T const* F) {
+  device_bound_ = true;
   eng_->bind_device(D, E, B, C, H, Cq, F);
   if constexpr (std::is_same_v<T, double>)
     if (mix_) mix_->engine().bind_device(D, E, B, C, H, Cq, F);
@@ -323,17 +328,20 @@ void BorderedCR<T>::bind_device(T const* D, T const* E, T const* B, T const* C, 
 
 template <typename T>
 int32_t BorderedCR<T>::factorize_device(bool sync) {
-  if (last_ != LastChoice::LU) throw std::runtime_error("factorize_device: only the LU last block is available on the GPU");
+  if (int(last_) > int(LastChoice::QRP))
+    throw std::runtime_error(std::string("factorize_device: ") + kLast[int(last_)] + " is not available on the GPU path");
   int32_t st = 0;
   if (mixed_) {
     if constexpr (std::is_same_v<T, double>) {
       auto& e = mix_->engine();
       e.set_method(int(sel_));
+      e.set_last_method(int(last_));
       e.factorize_async();
       if (sync) st = e.status_sync();
     }
   } else {
     eng_->set_method(int(sel_));
+    eng_->set_last_method(int(last_));
     eng_->factorize_async();
     if (sync) st = eng_->status_sync();
   }
@@ -341,24 +349,43 @@ int32_t BorderedCR<T>::factorize_device(bool sync) {
   return st;
 }
 
-// in-place solve of device columns; in MIXED mode each column is refined
+// in-place solve of device columns; in MIXED mode the columns are refined
+// together, and when refinement stalls the FP64 QRP path solves them again
 template <typename T>
 void BorderedCR<T>::solve_device(T* X, integer nrhs, int64_t ld, bool sync) {
   if (mixed_) {
     if constexpr (std::is_same_v<T, double>) {
-      report_ = bcr::RefineReport{};
-      for (integer c = 0; c < nrhs; ++c) {
-        bcr::RefineReport r = mix_->solve(X + size_t(c) * ld, ir_tol_, ir_max_);
-        report_.iterations = std::max(report_.iterations, r.iterations);
-        report_.rel_residual = std::max(report_.rel_residual, r.rel_residual);
-        report_.first_residual = std::max(report_.first_residual, r.first_residual);
-        report_.status = std::max(report_.status, r.status);
-      }
+      report_ = mix_->solve(X, nrhs, ld, ir_tol_, ir_max_);
+      if (report_.status != 0 && fallback_) fallback_qrp(X, nrhs, ld);
     }
     return;
   }
   eng_->template solve_async<T>(X, nrhs, ld);
   if (sync) bcr::hip_check(hipStreamSynchronize(eng_->stream()), "solve sync");
+}
+
+// FP64 cyclic reduction with column-pivoted QR on the pristine operator,
+// applied to the right-hand sides the refinement kept
+template <typename T>
+void BorderedCR<T>::fallback_qrp(T* X, integer nrhs, int64_t ld) {
+  if constexpr (std::is_same_v<T, double>) {
+    if (!device_bound_) eng_->upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
+    eng_->set_method(int(Choice::QRP));
+    eng_->factorize_async();
+    const int32_t st = eng_->status_sync();
+    eng_->set_method(int(sel_));
+    if (st != 0) {
+      report_.status = 3;
+      return;
+    }
+    const int64_t L = mix_->rhs_ld();
+    bcr::hip_check(hipMemcpy2DAsync(X, size_t(ld) * sizeof(double), mix_->rhs(), size_t(L) * sizeof(double),
+                                    size_t(L) * sizeof(double), nrhs, hipMemcpyDeviceToDevice, eng_->stream()),
+                   "fallback rhs");
+    eng_->template solve_async<double>(X, nrhs, ld);
+    report_.rel_residual = mix_->max_rel_residual(X, nrhs, ld, mix_->rhs(), L);
+    report_.status = 4;
+  }
 }
 
 template <typename T>

@@ -14,9 +14,8 @@
 //  * factorize() leaves the loaded blocks intact (the reference overwrites
 //    D, E, B, C, F, cc:1099-1153); Mv after factorize therefore still
 //    applies the original operator.
-//  * the last block supports LU (select_last_LU) on the GPU; the other
-//    select_last choices are accepted by name and report an error from
-//    factorize() until their GPU kernels exist.
+//  * the last block supports LU, LUPQ, QR and QRP on the GPU; SVD, LSS,
+//    LSY and PINV are accepted by name and make factorize() report an error.
 #pragma once
 
 #include <memory>
@@ -55,7 +54,18 @@ class BorderedCR {
   void select_QR() { sel_ = Choice::QR; }
   void select_QRP() { sel_ = Choice::QRP; }
   void select_last_LU() { last_ = LastChoice::LU; }
+  void select_last_LUPQ() { last_ = LastChoice::LUPQ; }
+  void select_last_QR() { last_ = LastChoice::QR; }
+  void select_last_QRP() { last_ = LastChoice::QRP; }
+  void select_last_SVD() { last_ = LastChoice::SVD; }
+  void select_last_LSS() { last_ = LastChoice::LSS; }
+  void select_last_LSY() { last_ = LastChoice::LSY; }
+  void select_last_PINV() { last_ = LastChoice::PINV; }
   void select_last2_NONE() { last2_ = LastChoice2::NONE; }
+  void select_last2_SVD() { last2_ = LastChoice2::SVD; }
+  void select_last2_LSS() { last2_ = LastChoice2::LSS; }
+  void select_last2_LSY() { last2_ = LastChoice2::LSY; }
+  void select_last2_PINV() { last2_ = LastChoice2::PINV; }
   Choice selected() const { return sel_; }
   LastChoice last_selected() const { return last_; }
   std::string info_algo() const;
@@ -145,6 +155,9 @@ class BorderedCR {
   // "MIXED" (double only): FP32 cyclic-reduction factors + FP64 iterative
   // refinement on the GPU to ||b - A x|| / ||b|| <= tol.
   void select_precision(std::string_view mode, double tol = 1e-12, int max_iter = 30);
+  // MIXED only: when refinement stalls, solve again with FP64 QRP elimination
+  // (report status 4); off: the solve reports the failure instead
+  void set_fallback(bool on) { fallback_ = on; }
   bool mixed() const { return mixed_; }
   bcr::RefineReport const& refine_report() const { return report_; }
   bcr::MixedEngine* mixed_engine() { return mix_.get(); }
@@ -166,6 +179,8 @@ class BorderedCR {
   bool mixed_ = false;
   double ir_tol_ = 1e-12;
   int ir_max_ = 30;
+  bool fallback_ = true, device_bound_ = false;
+  void fallback_qrp(T* X, integer nrhs, int64_t ld);
   bcr::RefineReport report_;
   bcr::DevBuf<T> xbuf_;
   mutable std::string last_error_{"no error"};

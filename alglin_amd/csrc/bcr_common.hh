@@ -69,6 +69,20 @@ struct AssembleArgs {
   double* Hm;                                 // Nr x Nc output
 };
 
+// Last-block factorization methods on the GPU (the select_last choices
+// LU, LUPQ, QR, QRP of BABD_BorderedCR.hxx:118-127, same numbering).
+enum LastMethod : int { kLastLU = 0, kLastLUPQ = 1, kLastQR = 2, kLastQRP = 3 };
+
+// Arguments of the last-block solve: Hm is the factored Nr x Nc block
+// (column-major, ld Nr); P = ipiv (LU, 1-based) / row swaps (LUPQ) / jpvt
+// (QRP, 0-based); Q = column swaps (LUPQ); tau = reflectors (QR, QRP).
+template <typename TX>
+struct LastSolveArgs {
+  int N, n, qr, qx, nr, nx, Nr, Nc, method, nrhs;
+  const double* Hm; const int32_t* P; const int32_t* Q; const double* tau;
+  TX* x; int64_t ldx;
+};
+
 // Arguments of the FP64 block SpMV (pristine operator).
 template <typename TA>
 struct MvArgs {

@@ -49,7 +49,15 @@ template <typename TA, typename TW>
 Engine<TA, TW>::~Engine() {
   for (auto& r : recs_) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : event_pool_) (void)hipEventDestroy(e);
-  if (st_) (void)hipStreamDestroy(st_);
+  if (st_ && own_stream_) (void)hipStreamDestroy(st_);
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::share_stream(hipStream_t s) {
+  hip_check(hipStreamSynchronize(st_), "share_stream sync");
+  if (own_stream_) hip_check(hipStreamDestroy(st_), "hipStreamDestroy");
+  st_ = s;
+  own_stream_ = false;
 }
 
 template <typename TA, typename TW>
@@ -108,6 +116,8 @@ void Engine<TA, TW>::allocate(Dims const& d) {
   Feff_.resize(std::max<size_t>(1, nr * nx));
   Hm_.resize(size_t(d.Nr()) * d.Nc());
   ipiv_.resize(std::max(d.Nr(), d.Nc()));
+  jpiv_.resize(std::max(d.Nr(), d.Nc()));
+  tau_.resize(std::max(d.Nr(), d.Nc()));
   status_.resize(4);
   tmpw_.resize(size_t(max_chunks_) * std::max<size_t>(1, nr * nx));
   tmpd_.resize(size_t(max_chunks_) * std::max<size_t>(1, nr * kRhsChunk));
@@ -185,7 +195,9 @@ void Engine<TA, TW>::factorize_async() {
   const int N = d_.N, n = d_.n, nr = d_.nr, nx = d_.nx;
   const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
   if (!pD_) throw HipError("factorize: no operator loaded");
-  if (d_.Nr() != d_.Nc()) throw HipError("factorize: the GPU last-block LU needs Nr == Nc (qr+nr == qx+nx)");
+  if ((last_method_ == kLastLU || last_method_ == kLastLUPQ) && d_.Nr() != d_.Nc())
+    throw HipError("factorize: the last-block LU needs Nr == Nc (qr+nr == qx+nx)");
+  if (d_.Nr() < d_.Nc()) throw HipError("factorize: the last block needs Nr >= Nc (qr+nr >= qx+nx)");
   // per-pair LDS tables of the generic and sweep kernels hold up to 64 pivots
   // and 16 border rows
   if (n > 64 || nr > 16) throw HipError("factorize: block size n <= 64 and nr <= 16 are supported");
@@ -231,7 +243,16 @@ void Engine<TA, TW>::factorize_async() {
   }
   timed(kLast, [&] {
     hip_check(launch_assemble_last(as, st_), "assemble last");
-    hip_check(launch_lu_last(Hm_.p, d_.Nc(), ipiv_.p, status_.p, lds_limit_, st_), "lu last");
+    switch (last_method_) {
+      case kLastLU: hip_check(launch_lu_last(Hm_.p, d_.Nc(), ipiv_.p, status_.p, lds_limit_, st_), "lu last"); break;
+      case kLastLUPQ:
+        hip_check(launch_lupq_last(Hm_.p, d_.Nc(), ipiv_.p, jpiv_.p, status_.p, lds_limit_, st_), "lupq last");
+        break;
+      default:
+        hip_check(launch_qr_last(last_method_ == kLastQRP, Hm_.p, d_.Nr(), d_.Nc(), tau_.p, ipiv_.p, status_.p,
+                                 lds_limit_, st_),
+                  "qr last");
+    }
   });
 }
 
@@ -270,7 +291,11 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
   while (L < N) {
     a.L = L;
     a.npairs = (N + L - 1) / (2 * L);
-    timed(kForward, [&] { hip_check(launch_forward(method_, a, st_), "forward"); });
+    timed(kForward, [&] {
+      hipError_t e = hipSuccess;
+      if (method_ == kLU && sweep_fast_ && launch_sweep_lu_fast(true, a, st_, &e)) hip_check(e, "forward_fast");
+      else hip_check(launch_forward(method_, a, st_), "forward");
+    });
     L *= 2;
   }
   if (N > 1 && m > 0)
@@ -280,14 +305,21 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
                 "reduce xb");
     });
   timed(kLast, [&] {
-    hip_check(launch_last_solve<TX>(Hm_.p, ipiv_.p, N, n, d_.qr, d_.qx, nr, d_.nx, x, ldx, nrhs, lds_limit_, st_),
-              "last solve");
+    LastSolveArgs<TX> ls{};
+    ls.N = N; ls.n = n; ls.qr = d_.qr; ls.qx = d_.qx; ls.nr = nr; ls.nx = d_.nx; ls.Nr = d_.Nr(); ls.Nc = d_.Nc();
+    ls.method = last_method_; ls.nrhs = nrhs; ls.Hm = Hm_.p; ls.P = ipiv_.p; ls.Q = jpiv_.p; ls.tau = tau_.p;
+    ls.x = x; ls.ldx = ldx;
+    hip_check(launch_last_solve<TX>(ls, lds_limit_, st_), "last solve");
   });
   int k = Lmax_;
   while ((k /= 2) > 0) {
     a.L = k;
     a.npairs = (N + k - 1) / (2 * k);
-    timed(kBackward, [&] { hip_check(launch_backward(method_, a, st_), "backward"); });
+    timed(kBackward, [&] {
+      hipError_t e = hipSuccess;
+      if (method_ == kLU && sweep_fast_ && launch_sweep_lu_fast(false, a, st_, &e)) hip_check(e, "backward_fast");
+      else hip_check(launch_backward(method_, a, st_), "backward");
+    });
   }
 }
 

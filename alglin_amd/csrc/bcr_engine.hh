@@ -67,8 +67,16 @@ class Engine {
   void allocate(Dims const& d);
   Dims const& dims() const { return d_; }
   hipStream_t stream() const { return st_; }
+  // run on another engine's stream (not owned) so that the two engines of one
+  // solver object order their work on a single stream
+  void share_stream(hipStream_t s);
 
   void set_method(int m) { method_ = m; }
+  // last-block factorization: kLastLU, kLastLUPQ, kLastQR, kLastQRP
+  void set_last_method(int m) { last_method_ = m; }
+  int last_method() const { return last_method_; }
+  // LU solves with n <= 32 use the wave-synchronous sweep kernels unless off
+  void set_sweep_fast(bool on) { sweep_fast_ = on; }
   int method() const { return method_; }
 
   // pristine operator: upload from host block arrays (reference layout)
@@ -120,7 +128,10 @@ class Engine {
   void level_factor(int L, bool first);
   Dims d_;
   int method_ = kLU;
+  int last_method_ = kLastLU;
+  bool sweep_fast_ = true;
   hipStream_t st_ = nullptr;
+  bool own_stream_ = true;
   size_t lds_limit_ = 65536;
   int max_chunks_ = 1024;
   FactorStats stats_;
@@ -131,8 +142,8 @@ class Engine {
            *pF_ = nullptr;
   // factors / working
   DevBuf<TW> wD_, wE_, wB_, wC_, Tm_, Cpend_, Fpart_, gscratch_, tmpw_;
-  DevBuf<int32_t> perm_, ipiv_, status_;
-  DevBuf<double> Feff_, Hm_, tmpd_, Cpart_, Csum_, Xpartd_;
+  DevBuf<int32_t> perm_, ipiv_, jpiv_, status_;
+  DevBuf<double> Feff_, Hm_, tau_, tmpd_, Cpart_, Csum_, Xpartd_;
   DevBuf<TW> Xpartw_;
   bool pend_clean_ = false;
 };

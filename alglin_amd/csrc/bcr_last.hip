@@ -109,54 +109,113 @@ __global__ void __launch_bounds__(1024) lu_last_kernel(double* __restrict__ Hm, 
     for (int i = tid; i < Nc * Nc; i += nth) Hm[i] = A[i];
 }
 
-template <typename TX>
-__global__ void __launch_bounds__(256) last_solve_kernel(const double* __restrict__ Hm,
-                                                          const int32_t* __restrict__ ipiv, int N, int n, int qr,
-                                                          int qx, int nr, int nx, TX* __restrict__ x, int64_t ldx,
-                                                          int nrhs) {
+// A_LDS: the factored last block is staged in LDS; otherwise (Nr*Nc*8 bytes
+// above the LDS limit, e.g. 152 x 152) it is read from global memory, where
+// it stays L2-resident, and only the right-hand-side chunk lives in LDS.
+// The Nr right-hand-side rows are gathered from x, the Nc solution rows are
+// scattered back.  Methods (kLastLU .. kLastQRP):
+//   LU   ipiv (1-based, row swaps), L then U            (getrs)
+//   LUPQ row swaps P, L, U, column swaps Q in reverse
+//   QR   y = Q^T b (reflectors in order), R^{-1} y      (ormqr + trsv)
+//   QRP  as QR, then x[jpvt[i]] = y[i]
+template <typename TX, bool A_LDS>
+__global__ void __launch_bounds__(1024) last_solve_kernel(LastSolveArgs<TX> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int Nc = 2 * n + qx + nx;  // == Nr
-  const int tid = threadIdx.x, nth = blockDim.x;
-  double* A = reinterpret_cast<double*>(smem_raw);
-  double* X = A + size_t(Nc) * Nc;
+  const int n = a.n, Nr = a.Nr, Nc = a.Nc, N = a.N, nrhs = a.nrhs, method = a.method;
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wid = tid >> 6;
+  const int ld = Nr;   // column-major Nr x Nc
+  const int R = Nr > Nc ? Nr : Nc;
+  double* sm = reinterpret_cast<double*>(smem_raw);
+  const double* A = A_LDS ? sm : a.Hm;
+  double* X = A_LDS ? sm + size_t(Nr) * Nc : sm;   // R x 16
+  double* Z = X + size_t(R) * 16;                   // Nc x 16 (QRP permutation)
   constexpr int QC = 16;
-  for (int i = tid; i < Nc * Nc; i += nth) A[i] = Hm[i];
+  if (A_LDS)
+    for (int i = tid; i < Nr * Nc; i += nth) sm[i] = a.Hm[i];
   const size_t xN = size_t(N) * n;
-  const size_t xC = size_t(N + 1) * n + qr;
+  const size_t xC = size_t(N + 1) * n + a.qr;
+  const int64_t ldx = a.ldx;
   for (int q0 = 0; q0 < nrhs; q0 += QC) {
     const int nq = min(QC, nrhs - q0);
-    for (int i = tid; i < Nc * nq; i += nth) {
-      const int r = i % Nc, q = i / Nc;
-      size_t row;
-      if (r < n) row = r;
-      else if (r < 2 * n + qr) row = xN + (r - n);
-      else row = xC + (r - 2 * n - qr);
-      X[r + q * Nc] = double(x[row + size_t(q0 + q) * ldx]);
+    for (int i = tid; i < R * nq; i += nth) {
+      const int r = i % R, q = i / R;
+      double v = 0;
+      if (r < Nr) {
+        size_t row;
+        if (r < n) row = r;
+        else if (r < 2 * n + a.qr) row = xN + (r - n);
+        else row = xC + (r - 2 * n - a.qr);
+        v = double(a.x[row + size_t(q0 + q) * ldx]);
+      }
+      X[r + q * R] = v;
     }
     __syncthreads();
-    for (int q = tid; q < nq; q += nth)
+    if (method == kLastLU || method == kLastLUPQ) {
+      for (int q = tid; q < nq; q += nth)
+        for (int c = 0; c < Nc; ++c) {
+          const int pv = method == kLastLU ? a.P[c] - 1 : a.P[c];
+          if (pv != c) { const double t = X[c + q * R]; X[c + q * R] = X[pv + q * R]; X[pv + q * R] = t; }
+        }
+      __syncthreads();
       for (int c = 0; c < Nc; ++c) {
-        const int pv = ipiv[c] - 1;
-        if (pv != c) { const double t = X[c + q * Nc]; X[c + q * Nc] = X[pv + q * Nc]; X[pv + q * Nc] = t; }
+        const int m = Nc - c - 1;
+        for (int i = tid; i < m * nq; i += nth) {
+          const int r = c + 1 + i % m, q = i / m;
+          X[r + q * R] -= A[r + c * ld] * X[c + q * R];
+        }
+        __syncthreads();
       }
-    __syncthreads();
-    for (int c = 0; c < Nc; ++c) {
-      const int m = Nc - c - 1;
-      for (int i = tid; i < m * nq; i += nth) {
-        const int r = c + 1 + i % m, q = i / m;
-        X[r + q * Nc] -= A[r + c * Nc] * X[c + q * Nc];
+    } else {
+      // y = Q^T b: H_0 first; wave q forms v' y for column q
+      for (int k = 0; k < Nc; ++k) {
+        const double tk = a.tau[k];
+        if (tk != 0.0) {
+          __shared__ double s_w[QC];
+          if (wid < nq) {
+            double s = 0;
+            for (int r = k + lane; r < Nr; r += 64) s += (r == k ? 1.0 : A[r + k * ld]) * X[r + wid * R];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) s_w[wid] = s * tk;
+          }
+          __syncthreads();
+          const int m = Nr - k;
+          for (int i = tid; i < m * nq; i += nth) {
+            const int r = k + i % m, q = i / m;
+            X[r + q * R] -= (r == k ? 1.0 : A[r + k * ld]) * s_w[q];
+          }
+          __syncthreads();
+        }
       }
+    }
+    // R^{-1} (or U^{-1}), bottom up
+    for (int c = Nc - 1; c >= 0; --c) {
+      const double uinv = 1.0 / A[c + c * ld];
+      for (int i = tid; i < c * nq; i += nth) {
+        const int r = i % c, q = i / c;
+        X[r + q * R] -= A[r + c * ld] * (X[c + q * R] * uinv);
+      }
+      __syncthreads();
+      for (int q = tid; q < nq; q += nth) X[c + q * R] *= uinv;
       __syncthreads();
     }
-    for (int c = Nc - 1; c >= 0; --c) {
-      const double uinv = 1.0 / A[c + c * Nc];
-      for (int i = tid; i < (c + 1) * nq; i += nth) {
-        const int r = i % (c + 1), q = i / (c + 1);
-        if (r < c) X[r + q * Nc] -= A[r + c * Nc] * (X[c + q * Nc] * uinv);
+    const double* Y = X;
+    int ldy = R;
+    if (method == kLastLUPQ) {
+      for (int q = tid; q < nq; q += nth)
+        for (int c = Nc - 1; c >= 0; --c) {
+          const int pv = a.Q[c];
+          if (pv != c) { const double t = X[c + q * R]; X[c + q * R] = X[pv + q * R]; X[pv + q * R] = t; }
+        }
+      __syncthreads();
+    } else if (method == kLastQRP) {
+      for (int i = tid; i < Nc * nq; i += nth) {
+        const int r = i % Nc, q = i / Nc;
+        Z[a.P[r] + q * Nc] = X[r + q * R];
       }
       __syncthreads();
-      for (int q = tid; q < nq; q += nth) X[c + q * Nc] *= uinv;
-      __syncthreads();
+      Y = Z;
+      ldy = Nc;
     }
     for (int i = tid; i < Nc * nq; i += nth) {
       const int r = i % Nc, q = i / Nc;
@@ -164,7 +223,7 @@ __global__ void __launch_bounds__(256) last_solve_kernel(const double* __restric
       if (r < n) row = r;
       else if (r < 2 * n) row = xN + (r - n);
       else row = size_t(N + 1) * n + (r - 2 * n);
-      x[row + size_t(q0 + q) * ldx] = TX(X[r + q * Nc]);
+      a.x[row + size_t(q0 + q) * ldx] = TX(Y[r + q * ldy]);
     }
     __syncthreads();
   }
@@ -186,22 +245,22 @@ hipError_t launch_lu_last(double* Hm, int Nc, int32_t* ipiv, int32_t* status, si
 }
 
 template <typename TX>
-hipError_t launch_last_solve(const double* Hm, const int32_t* ipiv, int N, int n, int qr, int qx, int nr, int nx,
-                             TX* x, int64_t ldx, int nrhs, size_t lds_limit, hipStream_t st) {
-  const int Nc = 2 * n + qx + nx;
-  const size_t bytes = (size_t(Nc) * Nc + size_t(Nc) * 16) * sizeof(double);
-  if (bytes > lds_limit) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL((last_solve_kernel<TX>), dim3(1), dim3(256), bytes, st, Hm, ipiv, N, n, qr, qx, nr, nx, x, ldx,
-                     nrhs);
+hipError_t launch_last_solve(LastSolveArgs<TX> const& a, size_t lds_limit, hipStream_t st) {
+  const int R = a.Nr > a.Nc ? a.Nr : a.Nc;
+  const size_t xbytes = (size_t(R) + a.Nc) * 16 * sizeof(double);
+  const size_t bytes = size_t(a.Nr) * a.Nc * sizeof(double) + xbytes;
+  if (xbytes > lds_limit) return hipErrorInvalidConfiguration;
+  if (bytes <= lds_limit)
+    hipLaunchKernelGGL((last_solve_kernel<TX, true>), dim3(1), dim3(1024), bytes, st, a);
+  else
+    hipLaunchKernelGGL((last_solve_kernel<TX, false>), dim3(1), dim3(1024), xbytes, st, a);
   return hipGetLastError();
 }
 
 template hipError_t launch_assemble_last<double, double>(AssembleArgs<double, double> const&, hipStream_t);
 template hipError_t launch_assemble_last<float, float>(AssembleArgs<float, float> const&, hipStream_t);
 template hipError_t launch_assemble_last<float, double>(AssembleArgs<float, double> const&, hipStream_t);
-template hipError_t launch_last_solve<double>(const double*, const int32_t*, int, int, int, int, int, int, double*,
-                                              int64_t, int, size_t, hipStream_t);
-template hipError_t launch_last_solve<float>(const double*, const int32_t*, int, int, int, int, int, int, float*,
-                                             int64_t, int, size_t, hipStream_t);
+template hipError_t launch_last_solve<double>(LastSolveArgs<double> const&, size_t, hipStream_t);
+template hipError_t launch_last_solve<float>(LastSolveArgs<float> const&, size_t, hipStream_t);
 
 }  // namespace bcr

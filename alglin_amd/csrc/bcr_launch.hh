@@ -21,6 +21,9 @@ template <typename TW, typename TX>
 hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
 template <typename TW, typename TX>
 hipError_t launch_backward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
+// wave-synchronous LU sweeps (n <= 32); false when the shape is not covered
+template <typename TW, typename TX>
+bool launch_sweep_lu_fast(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st, hipError_t* err);
 template <typename TP, typename TD>
 hipError_t launch_reduce(const TP* part, int first, int count, int m, TP* tmp, int max_chunks, TD* dst, int mrow,
                          int64_t ldd, hipStream_t st);
@@ -29,8 +32,11 @@ template <typename TW, typename TA>
 hipError_t launch_assemble_last(AssembleArgs<TW, TA> const& a, hipStream_t st);
 hipError_t launch_lu_last(double* Hm, int Nc, int32_t* ipiv, int32_t* status, size_t lds_limit, hipStream_t st);
 template <typename TX>
-hipError_t launch_last_solve(const double* Hm, const int32_t* ipiv, int N, int n, int qr, int qx, int nr, int nx,
-                             TX* x, int64_t ldx, int nrhs, size_t lds_limit, hipStream_t st);
+hipError_t launch_last_solve(LastSolveArgs<TX> const& a, size_t lds_limit, hipStream_t st);
+hipError_t launch_lupq_last(double* Hm, int Nc, int32_t* P, int32_t* Q, int32_t* status, size_t lds_limit,
+                            hipStream_t st);
+hipError_t launch_qr_last(bool pivot, double* Hm, int Nr, int Nc, double* tau, int32_t* jpvt, int32_t* status,
+                          size_t lds_limit, hipStream_t st);
 
 template <typename TA>
 hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st);

@@ -1,0 +1,228 @@
+// Wave-synchronous solve sweeps for LU-eliminated pairs, n <= 32.
+//
+// A pair is owned by a group of G = 2*NMAX lanes of one wavefront (NMAX in
+// {4, 8, 16, 32}; 64/G pairs per wavefront, four wavefronts per workgroup).
+// Lane r of the group holds row r of the pair's factor blocks in registers,
+// so the short recurrences (row interchanges, unit-lower and upper triangular
+// solves) run with cross-lane broadcasts (readlane when the group is the whole
+// wavefront, ds_bpermute otherwise) and no workgroup barrier.  Every
+// right-hand side of a chunk of QB is carried in its own register.
+//
+//   forward  (cc:1615-1719): W = [x_jp; x_j];  W <- P W;  W <- [L 0; M I]^{-1} W
+//            x_j <- W[0:n], x_jp <- W[n:2n]  (cc:1041-1042)
+//            Xpart[j] = -C_j' x_j            (cc:1644-1650)
+//   backward (cc:1805-1949): x_j <- U^{-1} (x_j - D_j x_jp - E_j x_jpp - B_j xb)
+//            (half 0 of the group forms D_j x_jp + B_j xb, half 1 E_j x_jpp)
+#include "bcr_launch.hh"
+
+namespace bcr {
+namespace {
+
+__device__ __forceinline__ double rlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ float rlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// value of lane base+idx of the group; for G == 64 idx must be wave-uniform
+template <int G, typename T>
+__device__ __forceinline__ T gbcast(T v, int base, int idx) {
+  if constexpr (G == 64) return rlane(v, idx);
+  else return __shfl(v, base + idx, 64);
+}
+
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename TW, typename TX, int NMAX, int QB>
+__global__ void __launch_bounds__(256) forward_lu_fast_kernel(SolveArgs<TW, TX> a) {
+  constexpr int G = 2 * NMAX, PG = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int r = lane % G, base = lane - r;
+  const int p = (int(blockIdx.x) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
+  const bool live = p < a.npairs;
+  const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
+  const int64_t ldx = a.ldx;
+  int j = 0, jp = 0, jpp = 0;
+  if (live) pair_index(p, a.L, a.N, j, jp, jpp);
+  const bool row = live && r < n2;
+
+  // row r of the 2n x n panel [L; M] (column-major, ld 2n)
+  const TW* Tj = a.Tm + size_t(j) * tsize(n);
+  TX t[NMAX];
+#pragma unroll
+  for (int c = 0; c < NMAX; ++c) t[c] = (row && c < n) ? TX(Tj[r + size_t(c) * n2]) : TX(0);
+
+  // row interchanges (laswp over rows 1..n with 1-based ipiv, cc:939):
+  // after the swaps row r holds the original row src
+  int src = r;
+  {
+    const int pvl = (live && r < n) ? a.perm[size_t(j) * n + r] - 1 : r;
+    for (int c = 0; c < n; ++c) {
+      const int pv = __shfl(pvl, base + c, 64);
+      const int sc = __shfl(src, base + c, 64);
+      const int sp = __shfl(src, base + pv, 64);
+      src = (r == c) ? sp : ((r == pv) ? sc : src);
+    }
+  }
+  const size_t in_row = r < n ? size_t(jp) * n + r : size_t(j) * n + (r - n);
+  const size_t out_row = r < n ? size_t(j) * n + r : size_t(jp) * n + (r - n);
+  const TW* Cj = a.C + size_t(j) * nr * n;
+
+  for (int q0 = 0; q0 < nrhs; q0 += QB) {
+    TX x[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      x[q] = (row && q0 + q < nrhs) ? a.x[in_row + size_t(q0 + q) * ldx] : TX(0);
+      x[q] = __shfl(x[q], base + src, 64);
+    }
+    // W <- [L 0; M I]^{-1} W : column c of the panel below the diagonal
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) {
+      if (c < n) {
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+          const TX xc = gbcast<G>(x[q], base, c);
+          x[q] = (r > c) ? x[q] - t[c] * xc : x[q];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+      if (row && q0 + q < nrhs) a.x[out_row + size_t(q0 + q) * ldx] = x[q];
+    // border partial -C_j' x_j over the top n rows of the group
+    for (int i = 0; i < nr; ++i) {
+      const TX ci = (live && r < n) ? TX(Cj[i + size_t(r) * nr]) : TX(0);
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        const TX v = group_sum<G>(ci * x[q]);
+        if (live && r == 0 && q0 + q < nrhs) a.Xpart[size_t(j) * nr * nrhs + i + size_t(q0 + q) * nr] = -v;
+      }
+    }
+  }
+}
+
+template <typename TW, typename TX, int NMAX, int QB>
+__global__ void __launch_bounds__(256) backward_lu_fast_kernel(SolveArgs<TW, TX> a) {
+  constexpr int G = 2 * NMAX, PG = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int r = lane % G, base = lane - r;
+  const int p = (int(blockIdx.x) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
+  const bool live = p < a.npairs;
+  const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
+  const int64_t ldx = a.ldx;
+  int j = 0, jp = 0, jpp = 0;
+  if (live) pair_index(p, a.L, a.N, j, jp, jpp);
+  const int h = r / NMAX, rr = r % NMAX;
+  const bool row = live && rr < n;
+  const size_t nn = size_t(n) * n;
+
+  // half 0: row rr of D_j and of U; half 1: row rr of E_j
+  const TW* M = (h == 0 ? a.D : a.E) + size_t(j) * nn;
+  const TW* Tj = a.Tm + size_t(j) * tsize(n);
+  TX m[NMAX], u[NMAX];
+#pragma unroll
+  for (int c = 0; c < NMAX; ++c) {
+    m[c] = (row && c < n) ? TX(M[rr + size_t(c) * n]) : TX(0);
+    u[c] = (row && h == 0 && c < n && c >= rr) ? TX(Tj[rr + size_t(c) * n2]) : TX(0);
+  }
+  const TW* Bj = a.B + size_t(j) * n * nx;
+  const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
+  const size_t vin_row = size_t(h == 0 ? jp : jpp) * n + rr;
+
+  for (int q0 = 0; q0 < nrhs; q0 += QB) {
+    TX s[QB], vin[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      vin[q] = (row && q0 + q < nrhs) ? a.x[vin_row + size_t(q0 + q) * ldx] : TX(0);
+      s[q] = TX(0);
+    }
+    // half 0: D_j x_jp, half 1: E_j x_jpp
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) {
+      if (c < n) {
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+          TX v;
+          if constexpr (G == 64) {
+            const TX v0 = rlane(vin[q], c), v1 = rlane(vin[q], NMAX + c);
+            v = h == 0 ? v0 : v1;
+          } else {
+            v = __shfl(vin[q], base + h * NMAX + c, 64);
+          }
+          s[q] += m[c] * v;
+        }
+      }
+    }
+    // half 0: + B_j xb
+    for (int c = 0; c < nx; ++c) {
+      const TX b = (row && h == 0) ? TX(Bj[rr + size_t(c) * n]) : TX(0);
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        const TX xb = (q0 + q < nrhs) ? a.x[xbrow + c + size_t(q0 + q) * ldx] : TX(0);
+        s[q] += b * xb;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const TX other = __shfl(s[q], base + NMAX + rr, 64);
+      const TX xj = (row && h == 0 && q0 + q < nrhs) ? a.x[size_t(j) * n + rr + size_t(q0 + q) * ldx] : TX(0);
+      s[q] = xj - s[q] - other;
+    }
+    // U^{-1}: column c, bottom up; lane c of half 0 owns the diagonal
+#pragma unroll
+    for (int c = NMAX - 1; c >= 0; --c) {
+      if (c < n) {
+        const TX den = rr == c ? u[c] : TX(1);
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+          const TX xc = gbcast<G>(s[q] / den, base, c);
+          s[q] = (rr < c) ? s[q] - u[c] * xc : ((rr == c) ? xc : s[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+      if (row && h == 0 && q0 + q < nrhs) a.x[size_t(j) * n + rr + size_t(q0 + q) * ldx] = s[q];
+  }
+}
+
+template <typename TW, typename TX, int NMAX>
+hipError_t launch_sweep_nmax(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st) {
+  constexpr int PG = 64 / (2 * NMAX);
+  const int blocks = (a.npairs + 4 * PG - 1) / (4 * PG);
+  if (forward) {
+    if (a.nrhs == 1) hipLaunchKernelGGL((forward_lu_fast_kernel<TW, TX, NMAX, 1>), dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((forward_lu_fast_kernel<TW, TX, NMAX, 4>), dim3(blocks), dim3(256), 0, st, a);
+  } else {
+    if (a.nrhs == 1) hipLaunchKernelGGL((backward_lu_fast_kernel<TW, TX, NMAX, 1>), dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((backward_lu_fast_kernel<TW, TX, NMAX, 4>), dim3(blocks), dim3(256), 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+template <typename TW, typename TX>
+bool launch_sweep_lu_fast(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st, hipError_t* err) {
+  if (a.n < 1 || a.n > 32 || a.npairs < 1) return false;
+  if (a.n <= 4) *err = launch_sweep_nmax<TW, TX, 4>(forward, a, st);
+  else if (a.n <= 8) *err = launch_sweep_nmax<TW, TX, 8>(forward, a, st);
+  else if (a.n <= 16) *err = launch_sweep_nmax<TW, TX, 16>(forward, a, st);
+  else *err = launch_sweep_nmax<TW, TX, 32>(forward, a, st);
+  return true;
+}
+
+template bool launch_sweep_lu_fast<double, double>(bool, SolveArgs<double, double> const&, hipStream_t, hipError_t*);
+template bool launch_sweep_lu_fast<float, float>(bool, SolveArgs<float, float> const&, hipStream_t, hipError_t*);
+template bool launch_sweep_lu_fast<float, double>(bool, SolveArgs<float, double> const&, hipStream_t, hipError_t*);
+
+}  // namespace bcr

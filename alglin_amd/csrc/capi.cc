@@ -295,6 +295,12 @@ int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32
   });
 }
 
+int alglin_bcr_set_fallback(alglin_bcr_t h, int on_) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) { s.set_fallback(on_ != 0); return 0; });
+  });
+}
+
 int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) {
@@ -393,16 +399,33 @@ void* alglin_dev_malloc(int64_t bytes) {
 
 int alglin_dev_free(void* p) { return p && hipFree(p) == hipSuccess ? 0 : 1; }
 
+// The solver streams are non-blocking (they do not order against the null
+// stream), so every plain copy below completes on the device before it
+// returns; work queued afterwards on any stream sees the data.
+static int copy_sync(void* dst, void const* src, int64_t bytes, hipMemcpyKind kind) {
+  if (bytes <= 0) return 0;
+  if (hipMemcpy(dst, src, size_t(bytes), kind) != hipSuccess) return 1;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 1;
+}
+
 int alglin_memcpy_htod(void* dst, void const* src, int64_t bytes) {
-  return hipMemcpy(dst, src, size_t(bytes), hipMemcpyHostToDevice) == hipSuccess ? 0 : 1;
+  return copy_sync(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
 int alglin_memcpy_dtoh(void* dst, void const* src, int64_t bytes) {
-  return hipMemcpy(dst, src, size_t(bytes), hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+  return copy_sync(dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
 int alglin_memcpy_dtod(void* dst, void const* src, int64_t bytes) {
-  return hipMemcpy(dst, src, size_t(bytes), hipMemcpyDeviceToDevice) == hipSuccess ? 0 : 1;
+  return copy_sync(dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+int alglin_memcpy_dtod_async(void* dst, void const* src, int64_t bytes, void* stream) {
+  if (bytes <= 0) return 0;
+  return hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)) ==
+                 hipSuccess
+             ? 0
+             : 1;
 }
 
 int alglin_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }

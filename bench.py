@@ -143,7 +143,7 @@ def main():
     def step(nrhs_buf=None, nrhs=1):
         buf = X if nrhs_buf is None else nrhs_buf[0]
         src = bd if nrhs_buf is None else nrhs_buf[1]
-        buf.copy_from(src)
+        buf.copy_from(src, stream=g.stream())
         if not g.factorize_device(sync=False):
             raise RuntimeError(g.last_error())
         g.solve_device(buf, nrhs, L, sync=True)

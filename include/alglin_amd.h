@@ -118,8 +118,13 @@ int alglin_bcr_fast_path(alglin_bcr_t h);
    only): FP32 cyclic-reduction factors + FP64 iterative refinement on the GPU
    until ||b - A x||_2 / ||b||_2 <= tol or max_iter correction sweeps */
 int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32_t max_iter);
+/* MIXED: when refinement stalls, solve again by FP64 cyclic reduction with
+   column-pivoted QR on the original operator (on by default); off = the solve
+   fails with "iterative refinement did not reach the requested residual" */
+int alglin_bcr_set_fallback(alglin_bcr_t h, int on);
 /* last refined solve: out = {sweeps, rel. residual, residual after the first
-   solve, status (0 converged, 1 stalled, 2 max_iter)} (worst over columns) */
+   solve, status (0 converged, 1 stalled, 2 max_iter, 3 factorization failed,
+   4 stalled and solved by the FP64 QRP fallback)} (worst over columns) */
 int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]);
 
 /* ---- per-phase GPU timing (HIP events on the handle's stream) --------------- */
@@ -135,9 +140,14 @@ int alglin_bcr_timing(alglin_bcr_t h, double out[8], int reset);
 /* ---- device memory (no other runtime needed) -------------------------------- */
 void* alglin_dev_malloc(int64_t bytes);
 int alglin_dev_free(void* p);
+/* plain copies return once the data is on the device (or host): work queued
+   afterwards on any stream, including a solver's stream, sees it */
 int alglin_memcpy_htod(void* dst, void const* src, int64_t bytes);
 int alglin_memcpy_dtoh(void* dst, void const* src, int64_t bytes);
 int alglin_memcpy_dtod(void* dst, void const* src, int64_t bytes);
+/* device copy queued on `stream` (e.g. alglin_bcr_stream(h)), ordered with the
+   solver's own work on that stream */
+int alglin_memcpy_dtod_async(void* dst, void const* src, int64_t bytes, void* stream);
 int alglin_device_synchronize(void);
 /* device name, compute units, HBM bytes */
 int alglin_device_info(char* name, int32_t len, int32_t* cus, int64_t* hbm_bytes);

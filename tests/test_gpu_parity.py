@@ -18,6 +18,9 @@ SHAPES = [
     (7, 2, 0, 0, 0, 0), (8, 4, 2, 2, 0, 0), (11, 5, 1, 3, 3, 1), (16, 1, 1, 1, 1, 1),
     (17, 6, 3, 3, 2, 2), (31, 4, 4, 4, 4, 4), (64, 3, 1, 1, 1, 1), (100, 2, 2, 2, 1, 1),
     (45, 8, 4, 4, 2, 2), (33, 16, 4, 4, 3, 3), (19, 32, 8, 8, 4, 4), (9, 40, 5, 5, 3, 3),
+    # last block 116 x 116 (staged in LDS) and 152 x 152 (read from global memory);
+    # n = 64 FP64 panels spill to the global scratch of the generic kernel
+    (5, 48, 12, 12, 8, 8), (6, 64, 16, 16, 8, 8),
 ]
 
 
