@@ -70,6 +70,17 @@ def lib():
             "alglin_memcpy_dtoh": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtod": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtod_async": (ci, [vp, vp, i64, vp]),
+            "alglin_batch_create": (vp, [ctypes.c_int]),
+            "alglin_batch_destroy": (None, [vp]),
+            "alglin_batch_plan": (ci, [vp, i32, vp]),
+            "alglin_batch_size": (i64, [vp, i32]),
+            "alglin_batch_offset": (i64, [vp, i32, i32]),
+            "alglin_batch_bind_device": (ci, [vp] * 8),
+            "alglin_batch_generate": (ci, [vp, ctypes.c_uint64, ctypes.c_double]),
+            "alglin_batch_solve": (ci, [vp, vp, vp, ctypes.c_double, i32, ctypes.c_int]),
+            "alglin_batch_report": (ci, [vp, vp, vp, vp, vp]),
+            "alglin_batch_last_error": (cs, [vp]),
+            "alglin_batch_stream": (vp, [vp]),
             "alglin_device_synchronize": (ci, []),
             "alglin_device_info": (ci, [ctypes.c_char_p, i32, vp, vp]),
             "BABD_factorize": (ci, [i32] * 7 + [vp, i32] * 4),
@@ -357,3 +368,123 @@ def packed_blocks(s: dict, dtype=np.float64) -> dict:
 
     return dict(D=blocks(s["D"]), E=blocks(s["E"]), B=blocks(s["B"]), C=blocks(s["C"]),
                 H=mat(s["H"]), Cq=mat(s["Cq"]), F=mat(s["F"]))
+
+
+def system_from_packed(N, n, qr, qx, nr, nx, D, E, B, C, H, Cq, F) -> dict:
+    """Inverse of packed_blocks: reference-layout arrays of one system -> system dict."""
+    Nc = 2 * n + qx + nx
+
+    def blocks(a, K, rows, cols):
+        return np.asarray(a).reshape(K, cols, rows).transpose(0, 2, 1).copy()
+
+    def mat(a, rows, cols):
+        return np.asarray(a).reshape(cols, rows).T.copy()
+
+    return dict(N=N, n=n, qr=qr, qx=qx, nr=nr, nx=nx,
+                D=blocks(D, N, n, n), E=blocks(E, N, n, n), B=blocks(B, N, n, nx), C=blocks(C, N + 1, nr, n),
+                H=mat(H, n + qr, Nc), Cq=mat(Cq, nr, qx), F=mat(F, nr, nx))
+
+
+class Batch:
+    """Many independent systems solved together (alglin_batch_* in alglin_amd.h).
+
+    dims: rows (N, n, qr, qx, nr, nx, method) with method 0 LU, 1 QR, 2 QRP.
+    The operator lives in packed device arrays owned by this object
+    (kinds DE, B, C, H, Cq, F); right-hand sides and solutions are packed
+    vectors of kind x."""
+
+    KINDS = ("DE", "B", "C", "H", "Cq", "F", "x")
+
+    def __init__(self, mixed: bool = False):
+        self._h = lib().alglin_batch_create(int(mixed))
+        if not self._h:
+            raise AlglinError("alglin_batch_create failed")
+        self.mixed = mixed
+        self.ops = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().alglin_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise AlglinError(f"{what}: {lib().alglin_batch_last_error(self._h).decode()}")
+
+    def last_error(self) -> str:
+        return lib().alglin_batch_last_error(self._h).decode()
+
+    def plan(self, dims):
+        d = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 7))
+        self.dims = d
+        self._check(lib().alglin_batch_plan(self._h, len(d), _p(d)), "plan")
+        # operator storage (device, packed)
+        if self.ops is not None:
+            for a in self.ops.values():
+                a.free()
+        sz = {k: max(1, int(lib().alglin_batch_size(self._h, i))) for i, k in enumerate(self.KINDS)}
+        self.ops = dict(D=DeviceArray(sz["DE"]), E=DeviceArray(sz["DE"]), B=DeviceArray(sz["B"]),
+                        C=DeviceArray(sz["C"]), H=DeviceArray(sz["H"]), Cq=DeviceArray(sz["Cq"]),
+                        F=DeviceArray(sz["F"]))
+        ptr = [ctypes.c_void_p(self.ops[k].data_ptr()) for k in ("D", "E", "B", "C", "H", "Cq", "F")]
+        self._check(lib().alglin_batch_bind_device(self._h, *ptr), "bind_device")
+
+    def size(self, kind: str) -> int:
+        return int(lib().alglin_batch_size(self._h, self.KINDS.index(kind)))
+
+    def offset(self, kind: str, s: int) -> int:
+        return int(lib().alglin_batch_offset(self._h, self.KINDS.index(kind), s))
+
+    def generate(self, seed: int, shift: float):
+        self._check(lib().alglin_batch_generate(self._h, ctypes.c_uint64(seed), shift), "generate")
+
+    def load_system(self, s: int, sysd: dict):
+        """Upload one system's blocks (system dict) into its packed slices."""
+        pk = packed_blocks(sysd)
+        for key, kind in (("D", "DE"), ("E", "DE"), ("B", "B"), ("C", "C"), ("H", "H"), ("Cq", "Cq"), ("F", "F")):
+            a = pk[key]
+            if a.size:
+                o = self.offset(kind, s) * 8
+                if lib().alglin_memcpy_htod(ctypes.c_void_p(self.ops[key].data_ptr() + o), _p(a), a.nbytes) != 0:
+                    raise AlglinError("H2D copy failed")
+
+    def system(self, s: int) -> dict:
+        """Download system s from the packed device arrays (system dict)."""
+        N, n, qr, qx, nr, nx, _ = (int(v) for v in self.dims[s])
+        Nc = 2 * n + qx + nx
+        cnt = dict(D=N * n * n, E=N * n * n, B=N * n * nx, C=(N + 1) * nr * n, H=(n + qr) * Nc, Cq=nr * qx,
+                   F=nr * nx)
+        kinds = dict(D="DE", E="DE", B="B", C="C", H="H", Cq="Cq", F="F")
+        out = {}
+        for key, c in cnt.items():
+            a = np.empty(c)
+            if c and lib().alglin_memcpy_dtoh(_p(a), ctypes.c_void_p(self.ops[key].data_ptr() +
+                                                                     8 * self.offset(kinds[key], s)), a.nbytes) != 0:
+                raise AlglinError("D2H copy failed")
+            out[key] = a
+        return system_from_packed(N, n, qr, qx, nr, nx, **out)
+
+    def solve(self, B: "DeviceArray", X: "DeviceArray", tol=1e-12, max_iter=30, graph=True) -> int:
+        """X = A^{-1} B per system; 0 all systems met tol, 2 some did not."""
+        rc = lib().alglin_batch_solve(self._h, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(X.data_ptr()), tol,
+                                      max_iter, int(graph))
+        if rc == 1:
+            raise AlglinError(f"batch solve: {self.last_error()}")
+        return rc
+
+    def report(self) -> dict:
+        ns = len(self.dims)
+        st = np.zeros(ns, np.int32)
+        rel = np.zeros(ns)
+        sw = np.zeros(ns, np.int32)
+        summ = np.zeros(4)
+        self._check(lib().alglin_batch_report(self._h, _p(st), _p(rel), _p(sw), _p(summ)), "report")
+        return dict(status=st, rel_residual=rel, sweeps=sw, max_sweeps=int(summ[0]), max_rel_residual=float(summ[1]),
+                    failed=int(summ[2]), ms=float(summ[3]))
+
+    def stream(self) -> int:
+        return lib().alglin_batch_stream(self._h)
+

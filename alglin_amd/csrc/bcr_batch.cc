@@ -1,0 +1,670 @@
+// Batched cyclic reduction (see bcr_batch.hh).
+#include "bcr_batch.hh"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace bcr {
+namespace {
+
+// host staging of the launch tables; uploaded once as one device blob
+struct Blob {
+  std::vector<char> h;
+  size_t add(const void* p, size_t bytes) {
+    const size_t off = (h.size() + 255) & ~size_t(255);
+    h.resize(off + std::max<size_t>(bytes, 1));
+    if (bytes) std::memcpy(h.data() + off, p, bytes);
+    return off;
+  }
+  template <typename T> size_t add(std::vector<T> const& v) { return add(v.data(), v.size() * sizeof(T)); }
+  void upload(DevBuf<char>& d, hipStream_t st) {
+    d.resize(std::max<size_t>(h.size(), 1));
+    hip_check(hipMemcpyAsync(d.p, h.data(), h.size(), hipMemcpyHostToDevice, st), "table upload");
+    hip_check(hipStreamSynchronize(st), "table upload sync");
+  }
+};
+
+int levels_of(int N) {
+  int k = 0;
+  for (int L = 1; L < N; L *= 2) ++k;
+  return k;
+}
+
+// factor kernel class of a system: 0..2 register kernels (n <= 4, 8, 16),
+// 3 two-wave kernel (16 < n <= 32), 4 LDS-panel LU, 5 QR, 6 QRP
+int factor_class(SysDims const& d) {
+  if (d.method == kQR) return 5;
+  if (d.method == kQRP) return 6;
+  if (d.n <= 4 && d.nx <= 4 && d.nr <= 4) return 0;
+  if (d.n <= 8 && d.nx <= 8 && d.nr <= 8) return 1;
+  if (d.n <= 16 && d.nx <= 8 && d.nr <= 8) return 2;
+  if (d.n <= 32 && d.nx <= 8 && d.nr <= 8) return 3;
+  return 4;
+}
+constexpr int kFacClasses = 7;
+constexpr int kFastNmax[3] = {4, 8, 16};
+
+// sweep kernel class: 0..3 wave kernels for LU with n <= 4, 8, 16, 32;
+// 4 generic LU, 5 generic QR, 6 generic QRP
+int sweep_class(SysDims const& d) {
+  if (d.method == kQR) return 5;
+  if (d.method == kQRP) return 6;
+  if (d.n <= 4) return 0;
+  if (d.n <= 8) return 1;
+  if (d.n <= 16) return 2;
+  if (d.n <= 32) return 3;
+  return 4;
+}
+constexpr int kSweepNmax[4] = {4, 8, 16, 32};
+
+}  // namespace
+
+template <typename TA, typename TW>
+BatchEngine<TA, TW>::BatchEngine() {
+  hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
+  int dev = 0, lds = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    lds_limit_ = size_t(lds);
+}
+
+template <typename TA, typename TW>
+BatchEngine<TA, TW>::~BatchEngine() {
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
+  if (dims.empty()) throw HipError("batch plan: no systems");
+  for (size_t s = 0; s < dims.size(); ++s) {
+    SysDims const& d = dims[s];
+    const std::string id = "batch plan: system " + std::to_string(s) + ": ";
+    if (d.N < 1 || d.n < 1 || d.qr < 0 || d.qx < 0 || d.nr < 0 || d.nx < 0) throw HipError(id + "bad dimensions");
+    if (d.n > 64 || d.nr > 16) throw HipError(id + "block size n <= 64 and nr <= 16 are supported");
+    if (d.Nr() != d.Nc()) throw HipError(id + "the last-block LU needs qr + nr == qx + nx");
+    if (d.method != kLU && d.method != kQR && d.method != kQRP) throw HipError(id + "method must be LU, QR or QRP");
+    const int c = factor_class(d);
+    if (c >= 4 && size_t(2 * d.n + d.nr) * (3 * d.n + d.nx) * sizeof(TW) > lds_limit_)
+      throw HipError(id + "pair panel does not fit in LDS");
+  }
+  dims_ = dims;
+  const int ns = nsys();
+  for (auto& v : off_) v.assign(ns + 1, 0);
+  for (auto* v : {&offT_, &offPerm_, &offCpend_, &offFpart_, &offHm_, &offPiv_, &offXpart_, &offCpart_, &offCsum_})
+    v->assign(ns + 1, 0);
+  maxN_ = Ncmax_ = Nrmax_ = 0;
+  for (int s = 0; s < ns; ++s) {
+    SysDims const& d = dims_[s];
+    const int64_t N = d.N, n = d.n, nr = d.nr, nx = d.nx;
+    off_[kbDE][s + 1] = off_[kbDE][s] + N * n * n;
+    off_[kbB][s + 1] = off_[kbB][s] + N * n * nx;
+    off_[kbC][s + 1] = off_[kbC][s] + (N + 1) * nr * n;
+    off_[kbH][s + 1] = off_[kbH][s] + (n + d.qr) * d.Nc();
+    off_[kbCq][s + 1] = off_[kbCq][s] + nr * d.qx;
+    off_[kbF][s + 1] = off_[kbF][s] + nr * nx;
+    off_[kbX][s + 1] = off_[kbX][s] + std::max(d.nrows(), d.ncols());
+    offT_[s + 1] = offT_[s] + N * int64_t(tsize(d.n));
+    offPerm_[s + 1] = offPerm_[s] + N * n;
+    offCpend_[s + 1] = offCpend_[s] + 2 * (N + 1) * nr * n;
+    offFpart_[s + 1] = offFpart_[s] + N * nr * nx;
+    offHm_[s + 1] = offHm_[s] + int64_t(d.Nr()) * d.Nc();
+    offPiv_[s + 1] = offPiv_[s] + std::max(d.Nr(), d.Nc());
+    offXpart_[s + 1] = offXpart_[s] + N * nr;
+    offCpart_[s + 1] = offCpart_[s] + (N + 1) * nr;
+    offCsum_[s + 1] = offCsum_[s] + nr;
+    maxN_ = std::max(maxN_, d.N);
+    Ncmax_ = std::max(Ncmax_, d.Nc());
+    Nrmax_ = std::max(Nrmax_, d.Nr());
+  }
+  for (int k = 0; k < kbKinds; ++k) tot_[k] = off_[k][ns];
+  levels_ = levels_of(maxN_);
+  auto one = [](int64_t v) { return size_t(std::max<int64_t>(v, 1)); };
+  wD_.resize(one(tot_[kbDE]));
+  wE_.resize(one(tot_[kbDE]));
+  wB_.resize(one(tot_[kbB]));
+  wC_.resize(one(tot_[kbC]));
+  Tm_.resize(one(offT_[ns]));
+  perm_.resize(one(offPerm_[ns]));
+  Cpend_.resize(one(offCpend_[ns]));
+  Fpart_.resize(one(offFpart_[ns]));
+  Feff_.resize(one(tot_[kbF]));
+  Hm_.resize(one(offHm_[ns]));
+  ipiv_.resize(one(offPiv_[ns]));
+  status_.resize(size_t(ns));
+  Xpart_.resize(one(offXpart_[ns]));
+  Cpart_.resize(one(offCpart_[ns]));
+  Csum_.resize(one(offCsum_[ns]));
+  factor_bytes_ = sizeof(TW) * (2 * tot_[kbDE] + tot_[kbB] + tot_[kbC] + offT_[ns] + offCpend_[ns] + offFpart_[ns]) +
+                  4 * offPerm_[ns] + 8 * offHm_[ns];
+  tables_ok_ = false;
+  solve_cache_.clear();
+  mv_cache_.clear();
+  norm_cache_.clear();
+  pD_ = pE_ = pB_ = pC_ = pH_ = pCq_ = pF_ = nullptr;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::bind_device(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq,
+                                      TA const* F) {
+  pD_ = D; pE_ = E; pB_ = B; pC_ = C; pH_ = H; pCq_ = Cq; pF_ = F;
+  tables_ok_ = false;
+  mv_cache_.clear();
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::generate(uint64_t seed, double shift) {
+  if constexpr (std::is_same_v<TA, double>) {
+    if (!pD_) throw HipError("batch generate: no operator bound");
+    std::vector<GenArgs> g(nsys());
+    for (int s = 0; s < nsys(); ++s) {
+      SysDims const& d = dims_[s];
+      GenArgs& a = g[s];
+      a.sys = s; a.N = d.N; a.n = d.n; a.qr = d.qr; a.qx = d.qx; a.nr = d.nr; a.nx = d.nx; a.shift = shift;
+      a.D = const_cast<double*>(pD_) + off_[kbDE][s];
+      a.E = const_cast<double*>(pE_) + off_[kbDE][s];
+      a.B = const_cast<double*>(pB_) + off_[kbB][s];
+      a.C = const_cast<double*>(pC_) + off_[kbC][s];
+      a.H = const_cast<double*>(pH_) + off_[kbH][s];
+      a.Cq = const_cast<double*>(pCq_) + off_[kbCq][s];
+      a.F = const_cast<double*>(pF_) + off_[kbF][s];
+    }
+    Blob bb;
+    const size_t o = bb.add(g);
+    DevBuf<char> blob;
+    bb.upload(blob, st_);
+    hip_check(launch_gen_batched(reinterpret_cast<const GenArgs*>(blob.p + o), nsys(), seed, st_), "generate");
+    hip_check(hipStreamSynchronize(st_), "generate sync");
+  } else {
+    throw HipError("batch generate: needs a double operator");
+  }
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::build_factor_tables() {
+  if (!pD_) throw HipError("batch factorize: no operator bound");
+  const int ns = nsys();
+  Blob bb;
+  struct Pending { size_t args, entry, first; Launch l; };
+  std::vector<std::vector<Pending>> pend(levels_);
+  int lev = 0;
+  for (int L = 1; L < maxN_; L *= 2, ++lev) {
+    int lg = 0;
+    for (int t = L; t > 1; t >>= 1) ++lg;
+    for (int cls = 0; cls < kFacClasses; ++cls) {
+      std::vector<FactorArgs<TA, TW>> a0;
+      std::vector<FactorArgs<TW, TW>> a1;
+      std::vector<int32_t> entry, first{0};
+      int nmax = 0, nxmax = 0, nrmax = 0;
+      std::vector<int> members;
+      for (int s = 0; s < ns; ++s) {
+        SysDims const& d = dims_[s];
+        if (d.N <= L || factor_class(d) != cls) continue;
+        members.push_back(s);
+        nmax = std::max(nmax, d.n);
+        nxmax = std::max(nxmax, d.nx);
+        nrmax = std::max(nrmax, d.nr);
+      }
+      if (members.empty()) continue;
+      int ppb = 1;
+      if (cls <= 2) ppb = pair_lu_fast_ppb(kFastNmax[cls], nxmax, nrmax);
+      for (int s : members) {
+        SysDims const& d = dims_[s];
+        FactorArgs<TW, TW> a{};
+        a.N = d.N; a.n = d.n; a.nr = d.nr; a.nx = d.nx; a.L = L;
+        a.npairs = (d.N + L - 1) / (2 * L);
+        a.slot_cur = lg & 1; a.first = L == 1 ? 1 : 0;
+        a.D = wD_.p + off_[kbDE][s]; a.E = wE_.p + off_[kbDE][s];
+        a.B = wB_.p + off_[kbB][s]; a.C = wC_.p + off_[kbC][s];
+        a.Tm = Tm_.p + offT_[s]; a.perm = perm_.p + offPerm_[s];
+        a.Cpend = Cpend_.p + offCpend_[s]; a.Fpart = Fpart_.p + offFpart_[s];
+        a.status = status_.p + s;
+        if (L == 1) {
+          FactorArgs<TA, TW> b{};
+          b.N = a.N; b.n = a.n; b.nr = a.nr; b.nx = a.nx; b.L = a.L; b.npairs = a.npairs;
+          b.slot_cur = a.slot_cur; b.first = 1;
+          b.sD = pD_ + off_[kbDE][s]; b.sE = pE_ + off_[kbDE][s]; b.sB = pB_ + off_[kbB][s]; b.sC = pC_ + off_[kbC][s];
+          b.D = a.D; b.E = a.E; b.B = a.B; b.C = a.C; b.Tm = a.Tm; b.perm = a.perm; b.Cpend = a.Cpend;
+          b.Fpart = a.Fpart; b.status = a.status;
+          a0.push_back(b);
+        } else {
+          a.sD = a.D; a.sE = a.E; a.sB = a.B; a.sC = a.C;
+          a1.push_back(a);
+        }
+        const int blocks = (a.npairs + ppb - 1) / ppb;
+        const int e = int(first.size()) - 1;
+        for (int b = 0; b < blocks; ++b) entry.push_back(e);
+        first.push_back(first.back() + blocks);
+      }
+      Pending p{};
+      p.args = L == 1 ? bb.add(a0) : bb.add(a1);
+      p.entry = bb.add(entry);
+      p.first = bb.add(first);
+      p.l.nblocks = first.back();
+      p.l.cls = cls;
+      p.l.method = cls == 5 ? kQR : cls == 6 ? kQRP : kLU;
+      p.l.nmax = cls <= 2 ? kFastNmax[cls] : nmax;
+      p.l.nxmax = nxmax;
+      p.l.nrmax = nrmax;
+      pend[lev].push_back(p);
+    }
+  }
+  // per-system records
+  std::vector<PrepArgs<TA, TW>> prep(ns);
+  std::vector<ReduceArgs<TW, double>> fred(ns);
+  std::vector<AssembleArgs<TW, TA>> as(ns);
+  std::vector<LastSolveArgs<double>> lu(ns);
+  std::vector<int32_t*> stp(ns);
+  for (int s = 0; s < ns; ++s) {
+    SysDims const& d = dims_[s];
+    PrepArgs<TA, TW>& p = prep[s];
+    p.N = d.N; p.n = d.n; p.nr = d.nr; p.nx = d.nx;
+    p.D = pD_ + off_[kbDE][s]; p.E = pE_ + off_[kbDE][s]; p.B = pB_ + off_[kbB][s]; p.C = pC_ + off_[kbC][s];
+    p.wD = wD_.p + off_[kbDE][s]; p.wE = wE_.p + off_[kbDE][s]; p.wB = wB_.p + off_[kbB][s];
+    p.wC = wC_.p + off_[kbC][s];
+    ReduceArgs<TW, double>& f = fred[s];
+    f.part = Fpart_.p + offFpart_[s]; f.first = 1; f.count = d.N - 1; f.m = d.nr * d.nx; f.mrow = std::max(1, f.m);
+    f.ldd = f.mrow; f.dst = Feff_.p + off_[kbF][s];
+    AssembleArgs<TW, TA>& a = as[s];
+    a.N = d.N; a.n = d.n; a.qr = d.qr; a.qx = d.qx; a.nr = d.nr; a.nx = d.nx; a.Nr = d.Nr(); a.Nc = d.Nc();
+    a.D0 = wD_.p + off_[kbDE][s]; a.E0 = wE_.p + off_[kbDE][s]; a.B0 = wB_.p + off_[kbB][s];
+    a.C = wC_.p + off_[kbC][s]; a.Cpend = Cpend_.p + offCpend_[s];
+    a.H = pH_ + off_[kbH][s]; a.Cq = pCq_ + off_[kbCq][s]; a.Feff = Feff_.p + off_[kbF][s];
+    a.Hm = Hm_.p + offHm_[s];
+    LastSolveArgs<double>& l = lu[s];
+    l = LastSolveArgs<double>{};
+    l.Nc = d.Nc(); l.Nr = d.Nr(); l.Hm = Hm_.p + offHm_[s]; l.P = ipiv_.p + offPiv_[s];
+    stp[s] = status_.p + s;
+  }
+  const size_t oprep = bb.add(prep), ofred = bb.add(fred), oasm = bb.add(as), olu = bb.add(lu), ost = bb.add(stp);
+  bb.upload(fblob_, st_);
+  char* base = fblob_.p;
+  fac_.assign(levels_, {});
+  for (int k = 0; k < levels_; ++k)
+    for (auto& p : pend[k]) {
+      Launch l = p.l;
+      l.args = base + p.args;
+      l.entry = reinterpret_cast<const int32_t*>(base + p.entry);
+      l.first = reinterpret_cast<const int32_t*>(base + p.first);
+      fac_[k].push_back(l);
+    }
+  prep_ = reinterpret_cast<const PrepArgs<TA, TW>*>(base + oprep);
+  fred_ = reinterpret_cast<const ReduceArgs<TW, double>*>(base + ofred);
+  asm_ = reinterpret_cast<const AssembleArgs<TW, TA>*>(base + oasm);
+  lu_ = reinterpret_cast<const LastSolveArgs<double>*>(base + olu);
+  stptr_ = reinterpret_cast<int32_t* const*>(base + ost);
+  tables_ok_ = true;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::factorize_async() {
+  if (!tables_ok_) build_factor_tables();
+  const int ns = nsys();
+  hip_check(launch_convert<int32_t, int32_t>(nullptr, status_.p, ns, st_), "status");
+  hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
+  hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
+  for (int k = 0; k < levels_; ++k)
+    for (Launch const& l : fac_[k]) {
+      hipError_t e = hipSuccess;
+      auto run = [&](auto const* args) {
+        using A = std::remove_cv_t<std::remove_pointer_t<decltype(args)>>;
+        BatchTable<A> t{args, l.entry, l.first};
+        bool ok = true;
+        if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
+        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
+        else {
+          const size_t bytes = size_t(2 * l.nmax + l.nrmax) * (3 * l.nmax + l.nxmax) * sizeof(TW);
+          e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, st_);
+        }
+        if (!ok) throw HipError("batch factorize: no kernel for a class");
+      };
+      if (k == 0) run(static_cast<const FactorArgs<TA, TW>*>(l.args));
+      else run(static_cast<const FactorArgs<TW, TW>*>(l.args));
+      hip_check(e, "batch pair elimination");
+    }
+  hip_check(launch_convert<TA, double>(pF_, Feff_.p, tot_[kbF], st_), "F");
+  hip_check(launch_reduce_batched<TW, double>(fred_, ns, st_), "reduce F");
+  hip_check(launch_assemble_last_batched<TW, TA>(asm_, ns, st_), "assemble last");
+  hip_check(launch_lu_last_batched(lu_, stptr_, ns, Ncmax_, lds_limit_, st_), "lu last");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::status(int32_t* host_out) {
+  hip_check(hipMemcpyAsync(host_out, status_.p, sizeof(int32_t) * nsys(), hipMemcpyDeviceToHost, st_), "status");
+  hip_check(hipStreamSynchronize(st_), "status sync");
+}
+
+template <typename TA, typename TW>
+typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(double* X) {
+  auto it = solve_cache_.find(X);
+  if (it != solve_cache_.end()) return it->second;
+  if (!tables_ok_) build_factor_tables();
+  SolveTables& T = solve_cache_[X];
+  const int ns = nsys();
+  Blob bb;
+  struct Pending { size_t args, entry, first; Launch l; };
+  std::vector<std::vector<Pending>> pend(levels_);
+  int lev = 0;
+  for (int L = 1; L < maxN_; L *= 2, ++lev) {
+    for (int cls = 0; cls < 7; ++cls) {
+      std::vector<SolveArgs<TW, double>> args;
+      std::vector<int32_t> entry, first{0};
+      int nmax = 0, nxmax = 0;
+      for (int s = 0; s < ns; ++s) {
+        SysDims const& d = dims_[s];
+        if (d.N <= L || sweep_class(d) != cls) continue;
+        SolveArgs<TW, double> a{};
+        a.N = d.N; a.n = d.n; a.nr = d.nr; a.nx = d.nx; a.qr = d.qr; a.qx = d.qx;
+        a.L = L; a.npairs = (d.N + L - 1) / (2 * L); a.nrhs = 1;
+        a.ldx = std::max(d.nrows(), d.ncols()); a.x = X + off_[kbX][s];
+        a.D = wD_.p + off_[kbDE][s]; a.E = wE_.p + off_[kbDE][s]; a.B = wB_.p + off_[kbB][s];
+        a.C = wC_.p + off_[kbC][s]; a.Tm = Tm_.p + offT_[s]; a.perm = perm_.p + offPerm_[s];
+        a.Xpart = Xpart_.p + offXpart_[s];
+        args.push_back(a);
+        nmax = std::max(nmax, d.n);
+        nxmax = std::max(nxmax, d.nx);
+        const int ppb = cls <= 3 ? sweep_lu_fast_pairs_per_block(kSweepNmax[cls]) : 1;
+        const int blocks = (a.npairs + ppb - 1) / ppb;
+        const int e = int(first.size()) - 1;
+        for (int b = 0; b < blocks; ++b) entry.push_back(e);
+        first.push_back(first.back() + blocks);
+      }
+      if (args.empty()) continue;
+      Pending p{};
+      p.args = bb.add(args);
+      p.entry = bb.add(entry);
+      p.first = bb.add(first);
+      p.l.nblocks = first.back();
+      p.l.cls = cls;
+      p.l.method = cls == 5 ? kQR : cls == 6 ? kQRP : kLU;
+      p.l.nmax = cls <= 3 ? kSweepNmax[cls] : nmax;
+      p.l.nxmax = nxmax;
+      pend[lev].push_back(p);
+    }
+  }
+  std::vector<ReduceArgs<double, double>> xb(ns);
+  std::vector<LastSolveArgs<double>> last(ns);
+  for (int s = 0; s < ns; ++s) {
+    SysDims const& d = dims_[s];
+    ReduceArgs<double, double>& r = xb[s];
+    r.part = Xpart_.p + offXpart_[s]; r.first = 1; r.count = d.N - 1; r.m = d.nr; r.mrow = std::max(1, d.nr);
+    r.ldd = std::max(d.nrows(), d.ncols());
+    r.dst = X + off_[kbX][s] + int64_t(d.N + 1) * d.n + d.qr;
+    LastSolveArgs<double>& l = last[s];
+    l.N = d.N; l.n = d.n; l.qr = d.qr; l.qx = d.qx; l.nr = d.nr; l.nx = d.nx; l.Nr = d.Nr(); l.Nc = d.Nc();
+    l.method = kLastLU; l.nrhs = 1; l.Hm = Hm_.p + offHm_[s]; l.P = ipiv_.p + offPiv_[s]; l.Q = nullptr;
+    l.tau = nullptr; l.x = X + off_[kbX][s]; l.ldx = std::max(d.nrows(), d.ncols());
+  }
+  const size_t oxb = bb.add(xb), olast = bb.add(last);
+  bb.upload(T.blob, st_);
+  char* base = T.blob.p;
+  T.fwd.assign(levels_, {});
+  for (int k = 0; k < levels_; ++k)
+    for (auto& p : pend[k]) {
+      Launch l = p.l;
+      l.args = base + p.args;
+      l.entry = reinterpret_cast<const int32_t*>(base + p.entry);
+      l.first = reinterpret_cast<const int32_t*>(base + p.first);
+      T.fwd[k].push_back(l);
+    }
+  T.xb = reinterpret_cast<const ReduceArgs<double, double>*>(base + oxb);
+  T.last = reinterpret_cast<const LastSolveArgs<double>*>(base + olast);
+  return T;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::solve_async(double* X) {
+  SolveTables& T = solve_tables(X);
+  const int ns = nsys();
+  auto sweep = [&](Launch const& l, bool fwd) {
+    BatchTable<SolveArgs<TW, double>> t{static_cast<const SolveArgs<TW, double>*>(l.args), l.entry, l.first};
+    hipError_t e = hipSuccess;
+    if (l.cls <= 3) {
+      if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, st_, &e))
+        throw HipError("batch solve: no sweep kernel");
+    } else {
+      e = launch_sweep_generic_batched(fwd, l.method, t, l.nblocks, l.nmax, l.nxmax, st_);
+    }
+    hip_check(e, fwd ? "batch forward" : "batch backward");
+  };
+  for (int k = 0; k < levels_; ++k)
+    for (Launch const& l : T.fwd[k]) sweep(l, true);
+  hip_check(launch_reduce_batched<double, double>(T.xb, ns, st_), "reduce xb");
+  hip_check(launch_last_solve_batched<double>(T.last, ns, Nrmax_, Ncmax_, lds_limit_, st_), "last solve");
+  for (int k = levels_ - 1; k >= 0; --k)
+    for (Launch const& l : T.fwd[k]) sweep(l, false);
+}
+
+template <typename TA, typename TW>
+typename BatchEngine<TA, TW>::MvTables& BatchEngine<TA, TW>::mv_tables(double const* X, double const* B, double* R) {
+  std::vector<const void*> key{X, B, R};
+  auto it = mv_cache_.find(key);
+  if (it != mv_cache_.end()) return it->second;
+  if (!pD_) throw HipError("batch residual: no operator bound");
+  MvTables& T = mv_cache_[key];
+  const int ns = nsys();
+  std::vector<MvArgs<TA>> mv(ns);
+  std::vector<int32_t> entry, first{0};
+  std::vector<ReduceArgs<double, double>> cs(ns);
+  std::vector<const double*> csp(ns);
+  for (int s = 0; s < ns; ++s) {
+    SysDims const& d = dims_[s];
+    MvArgs<TA>& m = mv[s];
+    m.N = d.N; m.n = d.n; m.qr = d.qr; m.qx = d.qx; m.nr = d.nr; m.nx = d.nx;
+    m.D = pD_ + off_[kbDE][s]; m.E = pE_ + off_[kbDE][s]; m.B = pB_ + off_[kbB][s]; m.C = pC_ + off_[kbC][s];
+    m.H = pH_ + off_[kbH][s]; m.Cq = pCq_ + off_[kbCq][s]; m.F = pF_ + off_[kbF][s];
+    m.x = X + off_[kbX][s]; m.b = B ? B + off_[kbX][s] : nullptr; m.out = R + off_[kbX][s];
+    m.Cpart = Cpart_.p + offCpart_[s];
+    const int blocks = (d.N + 1 + 3) / 4;
+    for (int b = 0; b < blocks; ++b) entry.push_back(s);
+    first.push_back(first.back() + blocks);
+    ReduceArgs<double, double>& r = cs[s];
+    r.part = Cpart_.p + offCpart_[s]; r.first = 0; r.count = d.N + 1; r.m = d.nr; r.mrow = std::max(1, d.nr);
+    r.ldd = r.mrow; r.dst = Csum_.p + offCsum_[s];
+    csp[s] = Csum_.p + offCsum_[s];
+  }
+  Blob bb;
+  const size_t om = bb.add(mv), oe = bb.add(entry), of = bb.add(first), oc = bb.add(cs), op = bb.add(csp);
+  bb.upload(T.blob, st_);
+  char* base = T.blob.p;
+  T.rows.args = base + om;
+  T.rows.entry = reinterpret_cast<const int32_t*>(base + oe);
+  T.rows.first = reinterpret_cast<const int32_t*>(base + of);
+  T.rows.nblocks = first.back();
+  T.csum = reinterpret_cast<const ReduceArgs<double, double>*>(base + oc);
+  T.tail = reinterpret_cast<const MvArgs<TA>*>(base + om);
+  T.csum_ptr = reinterpret_cast<const double* const*>(base + op);
+  return T;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::residual_async(double const* X, double const* B, double* R) {
+  MvTables& T = mv_tables(X, B, R);
+  const int ns = nsys();
+  BatchTable<MvArgs<TA>> t{static_cast<const MvArgs<TA>*>(T.rows.args), T.rows.entry, T.rows.first};
+  hip_check(launch_mv_rows_batched<TA>(t, T.rows.nblocks, st_), "batch mv rows");
+  hip_check(launch_convert<double, double>(nullptr, Csum_.p, int64_t(Csum_.count), st_), "zero Csum");
+  hip_check(launch_reduce_batched<double, double>(T.csum, ns, st_), "reduce Csum");
+  hip_check(launch_mv_tail_batched<TA>(T.tail, T.csum_ptr, ns, st_), "batch mv tail");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::prepare(double* X, double const* B, double* R) {
+  if (!tables_ok_) build_factor_tables();
+  solve_tables(X);
+  solve_tables(R);
+  mv_tables(X, B, R);
+  norms_async(B, R, nullptr);
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::norms_async(double const* B, double const* R, double* out) {
+  std::vector<const void*> key{B, R};
+  auto it = norm_cache_.find(key);
+  if (it == norm_cache_.end()) {
+    NormTables& T = norm_cache_[key];
+    std::vector<NormArgs> a(nsys());
+    for (int s = 0; s < nsys(); ++s) {
+      a[s].b = B + off_[kbX][s];
+      a[s].r = R + off_[kbX][s];
+      a[s].len = dims_[s].nrows();
+    }
+    Blob bb;
+    const size_t o = bb.add(a);
+    bb.upload(T.blob, st_);
+    T.args = reinterpret_cast<const NormArgs*>(T.blob.p + o);
+    it = norm_cache_.find(key);
+  }
+  if (out) hip_check(launch_norms_batched(it->second.args, nsys(), out, st_), "batch norms");
+}
+
+template class BatchEngine<double, double>;
+template class BatchEngine<double, float>;
+
+// ---------------------------------------------------------------------------
+BatchSolver::BatchSolver(bool mixed) : mixed_(mixed) {
+  if (mixed_) e32_ = new BatchEngine<double, float>();
+  else e64_ = new BatchEngine<double, double>();
+}
+
+BatchSolver::~BatchSolver() {
+  drop_graphs();
+  delete e64_;
+  delete e32_;
+}
+
+void BatchSolver::drop_graphs() {
+  for (hipGraphExec_t* g : {&g_.fs, &g_.res, &g_.corr})
+    if (*g) { (void)hipGraphExecDestroy(*g); *g = nullptr; }
+  g_.B = nullptr;
+  g_.X = nullptr;
+}
+
+hipStream_t BatchSolver::stream() const { return mixed_ ? e32_->stream() : e64_->stream(); }
+
+void BatchSolver::plan(std::vector<SysDims> const& dims) {
+  drop_graphs();
+  if (mixed_) e32_->plan(dims);
+  else e64_->plan(dims);
+  R_.resize(size_t(std::max<int64_t>(total(kbX), 1)));
+  norms_.resize(2 * dims.size());
+}
+
+int64_t BatchSolver::total(int kind) const { return mixed_ ? e32_->total(kind) : e64_->total(kind); }
+int64_t BatchSolver::offset(int kind, int s) const { return mixed_ ? e32_->offset(kind, s) : e64_->offset(kind, s); }
+
+void BatchSolver::bind_device(double const* D, double const* E, double const* B, double const* C, double const* H,
+                              double const* Cq, double const* F) {
+  drop_graphs();
+  if (mixed_) e32_->bind_device(D, E, B, C, H, Cq, F);
+  else e64_->bind_device(D, E, B, C, H, Cq, F);
+}
+
+void BatchSolver::generate(uint64_t seed, double shift) {
+  if (mixed_) e32_->generate(seed, shift);
+  else e64_->generate(seed, shift);
+}
+
+namespace {
+template <typename F>
+hipGraphExec_t capture(hipStream_t st, F&& body) {
+  hipGraph_t g = nullptr;
+  hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(st, &g), "end capture");
+  hipGraphExec_t ex = nullptr;
+  hip_check(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "graph instantiate");
+  (void)hipGraphDestroy(g);
+  return ex;
+}
+}  // namespace
+
+template <typename E>
+BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int ns = eng.nsys();
+  hipStream_t st = eng.stream();
+  const int64_t nx = eng.total(kbX);
+  auto factor_solve = [&] {
+    hip_check(hipMemcpyAsync(X, B, size_t(nx) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
+    eng.factorize_async();
+    eng.solve_async(X);
+  };
+  auto residual = [&] {
+    eng.residual_async(X, B, R_.p);
+    eng.norms_async(B, R_.p, norms_.p);
+  };
+  auto correct = [&] {
+    eng.solve_async(R_.p);
+    hip_check(launch_axpy(1.0, R_.p, X, nx, st), "x += d");
+  };
+  if (use_graph) {
+    if (g_.B != B || g_.X != X) {
+      drop_graphs();
+      eng.prepare(X, B, R_.p);   // no allocation inside a capture
+      g_.fs = capture(st, factor_solve);
+      g_.res = capture(st, residual);
+      g_.corr = capture(st, correct);
+      g_.B = B;
+      g_.X = X;
+    }
+    hip_check(hipGraphLaunch(g_.fs, st), "graph factor+solve");
+  } else {
+    factor_solve();
+  }
+  status_.assign(ns, 0);
+  eng.status(status_.data());
+  rel_.assign(ns, 0.0);
+  sweeps_.assign(ns, 0);
+  std::vector<double> best(ns, INFINITY), h(2 * size_t(ns));
+  std::vector<int> stagnant(ns, 0);
+  std::vector<char> done(ns, 0);
+  BatchReport rep;
+  for (int it = 0;; ++it) {
+    if (use_graph) hip_check(hipGraphLaunch(g_.res, st), "graph residual");
+    else residual();
+    hip_check(hipMemcpyAsync(h.data(), norms_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "norms");
+    hip_check(hipStreamSynchronize(st), "norms sync");
+    int open = 0;
+    for (int s = 0; s < ns; ++s) {
+      if (done[s]) continue;
+      const double nb = h[2 * s] > 0 ? h[2 * s] : 1.0;
+      const double rel = h[2 * s + 1] / nb;
+      rel_[s] = rel;
+      sweeps_[s] = it;
+      if (status_[s] != 0 || !(rel == rel) || rel <= tol) { done[s] = 1; continue; }
+      if (rel > 0.5 * best[s]) {
+        if (++stagnant[s] >= 3) { done[s] = 1; continue; }
+      } else {
+        stagnant[s] = 0;
+      }
+      best[s] = std::min(best[s], rel);
+      ++open;
+    }
+    rep.sweeps = it;
+    if (open == 0 || it >= max_iter) break;
+    if (use_graph) hip_check(hipGraphLaunch(g_.corr, st), "graph correction");
+    else correct();
+  }
+  for (int s = 0; s < ns; ++s) {
+    rep.max_rel_residual = std::max(rep.max_rel_residual, rel_[s]);
+    if (status_[s] == 0 && !(rel_[s] <= tol)) status_[s] |= 4;
+    if (status_[s] != 0) ++rep.failed;
+  }
+  rep.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rep;
+}
+
+BatchReport BatchSolver::solve(double const* B, double* X, double tol, int max_iter, bool use_graph) {
+  return mixed_ ? run(*e32_, B, X, tol, max_iter, use_graph) : run(*e64_, B, X, tol, max_iter, use_graph);
+}
+
+}  // namespace bcr

@@ -1,0 +1,186 @@
+// Batched bordered cyclic reduction: many independent BABD systems of uneven
+// block size n and block count N factored and solved together.
+//
+// Every system keeps the single-system algorithm (the reference's
+// BorderedCR::factorize / solve at num_parallel_block = 1,
+// BABD_BorderedCR.cc:803-839, 1489-1542).  What is batched is the launch:
+// at reduction level L every system with N > L has pairs to eliminate, and one
+// launch per kernel class covers the pairs of all of them (workgroup ->
+// (system, local block) through a per-level table built once by plan()).
+// Kernel classes follow the block size (register kernels for n <= 32, the
+// LDS-panel kernel above) and the elimination method of the system (LU, QR,
+// QRP), so systems of different shapes and methods share the same launches.
+// The last blocks, the border reductions and the residual SpMV run one
+// workgroup (or a few) per system in single launches.
+//
+// The whole factorize + solve sequence has a fixed launch list for a given
+// plan, so it is captured once into a HIP graph and replayed as one launch.
+//
+// Packed storage: array kind k of system s starts at element offset(k, s)
+// (kinds: D and E, B, C, H, Cq, F, x); blocks inside a slice follow the
+// reference layout (hxx:972-998).
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "bcr_batch_kernels.hh"
+#include "bcr_engine.hh"
+
+namespace bcr {
+
+struct SysDims {
+  int32_t N, n, qr, qx, nr, nx;
+  int32_t method;   // kLU, kQR, kQRP (elimination of the pairs)
+  int64_t nrows() const { return int64_t(n) * (N + 1) + qr + nr; }
+  int64_t ncols() const { return int64_t(n) * (N + 1) + qx + nx; }
+  int Nr() const { return 2 * n + qr + nr; }
+  int Nc() const { return 2 * n + qx + nx; }
+};
+
+enum BatchKind { kbDE = 0, kbB = 1, kbC = 2, kbH = 3, kbCq = 4, kbF = 5, kbX = 6, kbKinds = 7 };
+
+template <typename TA, typename TW>
+class BatchEngine {
+ public:
+  BatchEngine();
+  ~BatchEngine();
+  BatchEngine(BatchEngine const&) = delete;
+  BatchEngine& operator=(BatchEngine const&) = delete;
+
+  // validates the shapes, computes the packed offsets, allocates the factor
+  // storage; throws HipError on an unsupported shape
+  void plan(std::vector<SysDims> const& dims);
+  int nsys() const { return int(dims_.size()); }
+  SysDims const& dims(int s) const { return dims_[s]; }
+  int64_t total(int kind) const { return tot_[kind]; }
+  int64_t offset(int kind, int s) const { return off_[kind][s]; }
+
+  // the pristine operator (packed device arrays, kinds as above)
+  void bind_device(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq, TA const* F);
+  // fills the bound arrays with the synthetic Gaussian systems (needs TA = double)
+  void generate(uint64_t seed, double shift);
+
+  void factorize_async();
+  // per-system status after factorize: bit 0 zero pivot in a pair, bit 1 singular last block
+  void status(int32_t* host_out);
+  // in-place solve of the packed vector X (offsets kbX, 1 right-hand side)
+  void solve_async(double* X);
+  // R = B - A X per system (FP64, pristine operator)
+  void residual_async(double const* X, double const* B, double* R);
+  // out[2s] = ||B_s||, out[2s+1] = ||R_s|| (device array of 2*nsys)
+  void norms_async(double const* B, double const* R, double* out);
+
+  // builds every launch table these buffers need (no launch); a graph
+  // capture of factorize/solve/residual must not allocate
+  void prepare(double* X, double const* B, double* R);
+
+  hipStream_t stream() const { return st_; }
+  size_t factor_bytes() const { return factor_bytes_; }
+
+ private:
+  struct Launch {
+    const void* args;
+    const int32_t* entry;
+    const int32_t* first;
+    int nblocks, cls, method, nmax, nxmax, nrmax;
+  };
+  struct SolveTables {
+    std::vector<std::vector<Launch>> fwd;   // per level
+    std::vector<std::vector<Launch>> bwd;
+    const ReduceArgs<double, double>* xb = nullptr;
+    const LastSolveArgs<double>* last = nullptr;
+    DevBuf<char> blob;
+  };
+  struct MvTables {
+    Launch rows{};
+    const ReduceArgs<double, double>* csum = nullptr;
+    const MvArgs<TA>* tail = nullptr;
+    const double* const* csum_ptr = nullptr;
+    DevBuf<char> blob;
+  };
+  struct NormTables {
+    const NormArgs* args = nullptr;
+    DevBuf<char> blob;
+  };
+
+  void build_factor_tables();
+  SolveTables& solve_tables(double* X);
+  MvTables& mv_tables(double const* X, double const* B, double* R);
+
+  std::vector<SysDims> dims_;
+  std::vector<int64_t> off_[kbKinds];
+  int64_t tot_[kbKinds] = {0, 0, 0, 0, 0, 0, 0};
+  // factor-side offsets
+  std::vector<int64_t> offT_, offPerm_, offCpend_, offFpart_, offHm_, offPiv_, offXpart_, offCpart_, offCsum_;
+  int maxN_ = 0, levels_ = 0, Ncmax_ = 0, Nrmax_ = 0;
+  size_t lds_limit_ = 65536, factor_bytes_ = 0;
+  hipStream_t st_ = nullptr;
+
+  TA const *pD_ = nullptr, *pE_ = nullptr, *pB_ = nullptr, *pC_ = nullptr, *pH_ = nullptr, *pCq_ = nullptr,
+           *pF_ = nullptr;
+  DevBuf<TW> wD_, wE_, wB_, wC_, Tm_, Cpend_, Fpart_;
+  DevBuf<int32_t> perm_, ipiv_, status_;
+  DevBuf<double> Feff_, Hm_, Xpart_, Cpart_, Csum_;
+
+  // factor tables
+  std::vector<std::vector<Launch>> fac_;   // per level
+  const PrepArgs<TA, TW>* prep_ = nullptr;
+  const ReduceArgs<TW, double>* fred_ = nullptr;
+  const AssembleArgs<TW, TA>* asm_ = nullptr;
+  const LastSolveArgs<double>* lu_ = nullptr;
+  int32_t* const* stptr_ = nullptr;
+  DevBuf<char> fblob_;
+  bool tables_ok_ = false;
+
+  std::map<const void*, SolveTables> solve_cache_;
+  std::map<std::vector<const void*>, MvTables> mv_cache_;
+  std::map<std::vector<const void*>, NormTables> norm_cache_;
+};
+
+// The batched solver a caller sees: FP64 cyclic reduction, or FP32 factors
+// with FP64 iterative refinement per system (all systems swept together,
+// until every system reaches tol or stalls).
+struct BatchReport {
+  int sweeps = 0;              // refinement sweeps after the first solve
+  double max_rel_residual = 0; // over systems
+  int failed = 0;              // systems that did not reach tol
+  double ms_total = 0;         // wall time of the solve call (ms)
+};
+
+class BatchSolver {
+ public:
+  explicit BatchSolver(bool mixed);
+  ~BatchSolver();
+  void plan(std::vector<SysDims> const& dims);
+  int64_t total(int kind) const;
+  int64_t offset(int kind, int s) const;
+  void bind_device(double const* D, double const* E, double const* B, double const* C, double const* H,
+                   double const* Cq, double const* F);
+  void generate(uint64_t seed, double shift);
+  // B, X: packed device vectors (kind x); X = A^{-1} B per system
+  BatchReport solve(double const* B, double* X, double tol, int max_iter, bool use_graph);
+  std::vector<int32_t> const& status() const { return status_; }
+  std::vector<double> const& rel_residual() const { return rel_; }
+  std::vector<int32_t> const& sweeps() const { return sweeps_; }
+  hipStream_t stream() const;
+  bool mixed() const { return mixed_; }
+
+ private:
+  template <typename E> BatchReport run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph);
+  bool mixed_;
+  BatchEngine<double, double>* e64_ = nullptr;
+  BatchEngine<double, float>* e32_ = nullptr;
+  DevBuf<double> R_, norms_;
+  std::vector<int32_t> status_, sweeps_;
+  std::vector<double> rel_;
+  // captured graphs: factor+solve, residual+norms, correction
+  struct Graphs {
+    const double* B = nullptr;
+    double* X = nullptr;
+    hipGraphExec_t fs = nullptr, res = nullptr, corr = nullptr;
+  } g_;
+  void drop_graphs();
+};
+
+}  // namespace bcr

@@ -1,0 +1,88 @@
+// Argument records and launchers of the batched (many systems per launch)
+// kernels.  Each record's pointers are already offset to one system's slice
+// of the packed arrays.
+#pragma once
+
+#include "bcr_launch.hh"
+
+namespace bcr {
+
+template <typename TA, typename TW>
+struct PrepArgs {
+  int N, n, nr, nx;
+  const TA *D, *E, *B, *C;
+  TW *wD, *wE, *wB, *wC;
+};
+
+// dst[(i % mrow) + (i / mrow) * ldd] += sum_{k=first}^{first+count-1} part[k*m + i]
+template <typename TP, typename TD>
+struct ReduceArgs {
+  const TP* part;
+  int first, count, m, mrow;
+  int64_t ldd;
+  TD* dst;
+};
+
+struct NormArgs {
+  const double* b;
+  const double* r;
+  int64_t len;
+};
+
+// synthetic system s: D_k = shift I + G/sqrt(n), E_k = G/sqrt(n), B, C, H, Cq, F = G
+struct GenArgs {
+  int sys, N, n, qr, qx, nr, nx;
+  double shift;
+  double *D, *E, *B, *C, *H, *Cq, *F;
+};
+
+template <typename TA, typename TW>
+hipError_t launch_prep_batched(const PrepArgs<TA, TW>* args, int nsys, hipStream_t st);
+template <typename TP, typename TD>
+hipError_t launch_reduce_batched(const ReduceArgs<TP, TD>* args, int nsys, hipStream_t st);
+hipError_t launch_norms_batched(const NormArgs* args, int nsys, double* out, hipStream_t st);
+hipError_t launch_gen_batched(const GenArgs* args, int nsys, uint64_t seed, hipStream_t st);
+
+// pair elimination (factor), one launch per reduction level and kernel class
+inline int pair_lu_fast_ppb(int n, int nx, int nr) {
+  if (n <= 4 && nx <= 4 && nr <= 4) return kWave / 8;
+  if (n <= 8 && nx <= 8 && nr <= 8) return kWave / 16;
+  if (n <= 16 && nx <= 8 && nr <= 8) return kWave / 32;
+  if (n <= 32 && nx <= 8 && nr <= 8) return 1;
+  return 0;
+}
+template <typename TS, typename TW>
+bool launch_pair_lu_fast_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
+                                 hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
+bool launch_pair_lu_split_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
+                                  hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
+hipError_t launch_pair_generic_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, size_t bytes,
+                                       size_t lds_limit, hipStream_t st);
+
+// sweeps
+int sweep_lu_fast_pairs_per_block(int n);
+template <typename TW, typename TX>
+bool launch_sweep_lu_fast_batched(bool forward, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks, int n, int nrhs,
+                                  hipStream_t st, hipError_t* err);
+template <typename TW, typename TX>
+hipError_t launch_sweep_generic_batched(bool forward, int method, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks,
+                                        int n, int nx, hipStream_t st);
+
+// last block, one workgroup per system
+template <typename TW, typename TA>
+hipError_t launch_assemble_last_batched(const AssembleArgs<TW, TA>* args, int nsys, hipStream_t st);
+hipError_t launch_lu_last_batched(const LastSolveArgs<double>* args, int32_t* const* status, int nsys, int Ncmax,
+                                  size_t lds_limit, hipStream_t st);
+template <typename TX>
+hipError_t launch_last_solve_batched(const LastSolveArgs<TX>* args, int nsys, int Nrmax, int Ncmax, size_t lds_limit,
+                                     hipStream_t st);
+
+// residual SpMV
+template <typename TA>
+hipError_t launch_mv_rows_batched(BatchTable<MvArgs<TA>> const& t, int nblocks, hipStream_t st);
+template <typename TA>
+hipError_t launch_mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum, int nsys, hipStream_t st);
+
+}  // namespace bcr

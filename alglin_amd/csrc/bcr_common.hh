@@ -100,6 +100,16 @@ __host__ __device__ __forceinline__ size_t align16(size_t bytes) { return (bytes
 template <typename T> __device__ __forceinline__ T dev_abs(T v) { return v < T(0) ? -v : v; }
 
 // pair p of level L (pure cyclic reduction over [0,N), cc:1088-1133)
+// Batched launches: workgroup b serves entry e = entry[b] (one system at one
+// level) as its local block b - first[e]; args[e] holds that system's
+// arguments (pointers already offset to the system's slices).
+template <typename Args>
+struct BatchTable {
+  const Args* args;
+  const int32_t* entry;
+  const int32_t* first;
+};
+
 __device__ __forceinline__ void pair_index(int p, int L, int N, int& j, int& jp, int& jpp) {
   j = L + 2 * L * p;
   jp = j - L;

@@ -60,15 +60,14 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 template <typename TS, typename TW, int METHOD, bool IN_LDS>
-__global__ void __launch_bounds__(256)
-pair_generic_kernel(FactorArgs<TS, TW> a, TW* __restrict__ gscratch) {
+__device__ __forceinline__ void pair_generic_body(FactorArgs<TS, TW> a, TW* __restrict__ gscratch, int bid) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ TW s_vn1[128], s_vn2[128], s_tau[64], s_lr[16];
   __shared__ TW s_red[4];
   __shared__ int s_piv;
   __shared__ int s_jpvt[64];
 
-  const int p = blockIdx.x;
+  const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
   pair_index(p, a.L, a.N, j, jp, jpp);
@@ -320,6 +319,12 @@ pair_generic_kernel(FactorArgs<TS, TW> a, TW* __restrict__ gscratch) {
   }
 }
 
+template <typename TS, typename TW, int METHOD, bool IN_LDS>
+__global__ void __launch_bounds__(256) pair_generic_kernel(FactorArgs<TS, TW> a, TW* __restrict__ gscratch) {
+  pair_generic_body<TS, TW, METHOD, IN_LDS>(a, gscratch, blockIdx.x);
+}
+
+
 template <typename TS, typename TW>
 hipError_t launch_pair_generic(int method, FactorArgs<TS, TW> const& a, TW* gscratch, size_t lds_limit,
                                hipStream_t st) {
@@ -338,6 +343,29 @@ hipError_t launch_pair_generic(int method, FactorArgs<TS, TW> const& a, TW* gscr
   return hipGetLastError();
 }
 
+template <typename TS, typename TW, int METHOD>
+__global__ void __launch_bounds__(256) pair_generic_batched(BatchTable<FactorArgs<TS, TW>> t) {
+  const int e = t.entry[blockIdx.x];
+  pair_generic_body<TS, TW, METHOD, true>(t.args[e], nullptr, int(blockIdx.x) - t.first[e]);
+}
+
+// panels staged in LDS only: `bytes` is the largest panel of the launch
+template <typename TS, typename TW>
+hipError_t launch_pair_generic_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, size_t bytes,
+                                       size_t lds_limit, hipStream_t st) {
+  if (bytes > lds_limit) return hipErrorInvalidValue;
+  if (method == kLU) hipLaunchKernelGGL((pair_generic_batched<TS, TW, kLU>), dim3(nblocks), dim3(256), bytes, st, t);
+  else if (method == kQR) hipLaunchKernelGGL((pair_generic_batched<TS, TW, kQR>), dim3(nblocks), dim3(256), bytes, st, t);
+  else hipLaunchKernelGGL((pair_generic_batched<TS, TW, kQRP>), dim3(nblocks), dim3(256), bytes, st, t);
+  return hipGetLastError();
+}
+
+template hipError_t launch_pair_generic_batched<double, double>(int, BatchTable<FactorArgs<double, double>> const&, int,
+                                                                size_t, size_t, hipStream_t);
+template hipError_t launch_pair_generic_batched<float, float>(int, BatchTable<FactorArgs<float, float>> const&, int,
+                                                              size_t, size_t, hipStream_t);
+template hipError_t launch_pair_generic_batched<double, float>(int, BatchTable<FactorArgs<double, float>> const&, int,
+                                                               size_t, size_t, hipStream_t);
 template hipError_t launch_pair_generic<double, double>(int, FactorArgs<double, double> const&, double*, size_t, hipStream_t);
 template hipError_t launch_pair_generic<float, float>(int, FactorArgs<float, float> const&, float*, size_t, hipStream_t);
 template hipError_t launch_pair_generic<double, float>(int, FactorArgs<double, float> const&, float*, size_t, hipStream_t);

@@ -3,4 +3,6 @@
 
 namespace bcr {
 template bool launch_pair_lu_split<double, double>(FactorArgs<double, double> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_split_batched<double, double>(BatchTable<FactorArgs<double, double>> const&, int, int, int, int, hipStream_t,
+                                              hipError_t*);
 }  // namespace bcr

@@ -3,4 +3,6 @@
 
 namespace bcr {
 template bool launch_pair_lu_split<float, float>(FactorArgs<float, float> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_split_batched<float, float>(BatchTable<FactorArgs<float, float>> const&, int, int, int, int, hipStream_t,
+                                              hipError_t*);
 }  // namespace bcr

@@ -21,7 +21,7 @@ namespace bcr {
 
 
 template <typename TW, typename TA>
-__global__ void __launch_bounds__(256) assemble_last_kernel(AssembleArgs<TW, TA> a) {
+__device__ __forceinline__ void assemble_last_body(AssembleArgs<TW, TA> const& a) {
   const int n = a.n, Nr = a.Nr, Nc = a.Nc, qr = a.qr, qx = a.qx, nr = a.nr, nx = a.nx;
   const int tid = threadIdx.x, nth = blockDim.x;
   const size_t rn = size_t(nr) * n;
@@ -51,11 +51,16 @@ __global__ void __launch_bounds__(256) assemble_last_kernel(AssembleArgs<TW, TA>
   }
 }
 
+template <typename TW, typename TA>
+__global__ void __launch_bounds__(256) assemble_last_kernel(AssembleArgs<TW, TA> a) {
+  assemble_last_body<TW, TA>(a);
+}
+
 // LU with partial pivoting (dgetf2 order: first max, whole-row swaps,
 // reciprocal scaling) of the square Nc x Nc last block.
 template <bool IN_LDS>
-__global__ void __launch_bounds__(1024) lu_last_kernel(double* __restrict__ Hm, int Nc, int32_t* __restrict__ ipiv,
-                                                        int32_t* __restrict__ status) {
+__device__ __forceinline__ void lu_last_body(double* __restrict__ Hm, int Nc, int32_t* __restrict__ ipiv,
+                                             int32_t* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int s_piv;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -118,8 +123,14 @@ __global__ void __launch_bounds__(1024) lu_last_kernel(double* __restrict__ Hm, 
 //   LUPQ row swaps P, L, U, column swaps Q in reverse
 //   QR   y = Q^T b (reflectors in order), R^{-1} y      (ormqr + trsv)
 //   QRP  as QR, then x[jpvt[i]] = y[i]
+template <bool IN_LDS>
+__global__ void __launch_bounds__(1024) lu_last_kernel(double* __restrict__ Hm, int Nc, int32_t* __restrict__ ipiv,
+                                                        int32_t* __restrict__ status) {
+  lu_last_body<IN_LDS>(Hm, Nc, ipiv, status);
+}
+
 template <typename TX, bool A_LDS>
-__global__ void __launch_bounds__(1024) last_solve_kernel(LastSolveArgs<TX> a) {
+__device__ __forceinline__ void last_solve_body(LastSolveArgs<TX> const& a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int n = a.n, Nr = a.Nr, Nc = a.Nc, N = a.N, nrhs = a.nrhs, method = a.method;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wid = tid >> 6;
@@ -228,6 +239,64 @@ __global__ void __launch_bounds__(1024) last_solve_kernel(LastSolveArgs<TX> a) {
     __syncthreads();
   }
 }
+
+template <typename TX, bool A_LDS>
+__global__ void __launch_bounds__(1024) last_solve_kernel(LastSolveArgs<TX> a) {
+  last_solve_body<TX, A_LDS>(a);
+}
+
+// ---- batched: one workgroup per system ------------------------------------
+template <typename TW, typename TA>
+__global__ void __launch_bounds__(256) assemble_last_batched(const AssembleArgs<TW, TA>* args) {
+  assemble_last_body<TW, TA>(args[blockIdx.x]);
+}
+
+template <bool IN_LDS>
+__global__ void __launch_bounds__(1024) lu_last_batched(const LastSolveArgs<double>* args, int32_t* const* status) {
+  const LastSolveArgs<double>& a = args[blockIdx.x];
+  lu_last_body<IN_LDS>(const_cast<double*>(a.Hm), a.Nc, const_cast<int32_t*>(a.P), status[blockIdx.x]);
+}
+
+template <typename TX, bool A_LDS>
+__global__ void __launch_bounds__(1024) last_solve_batched(const LastSolveArgs<TX>* args) {
+  last_solve_body<TX, A_LDS>(args[blockIdx.x]);
+}
+
+template <typename TW, typename TA>
+hipError_t launch_assemble_last_batched(const AssembleArgs<TW, TA>* args, int nsys, hipStream_t st) {
+  hipLaunchKernelGGL((assemble_last_batched<TW, TA>), dim3(nsys), dim3(256), 0, st, args);
+  return hipGetLastError();
+}
+
+// args[s].Hm / .P (ipiv) / .Nc describe the square last block of system s;
+// Ncmax is the largest of the launch
+hipError_t launch_lu_last_batched(const LastSolveArgs<double>* args, int32_t* const* status, int nsys, int Ncmax,
+                                  size_t lds_limit, hipStream_t st) {
+  const size_t bytes = size_t(Ncmax) * Ncmax * sizeof(double);
+  if (bytes <= lds_limit)
+    hipLaunchKernelGGL((lu_last_batched<true>), dim3(nsys), dim3(1024), bytes, st, args, status);
+  else
+    hipLaunchKernelGGL((lu_last_batched<false>), dim3(nsys), dim3(1024), 0, st, args, status);
+  return hipGetLastError();
+}
+
+template <typename TX>
+hipError_t launch_last_solve_batched(const LastSolveArgs<TX>* args, int nsys, int Nrmax, int Ncmax, size_t lds_limit,
+                                     hipStream_t st) {
+  const int R = Nrmax > Ncmax ? Nrmax : Ncmax;
+  const size_t xbytes = (size_t(R) + Ncmax) * 16 * sizeof(double);
+  const size_t bytes = size_t(Nrmax) * Ncmax * sizeof(double) + xbytes;
+  if (xbytes > lds_limit) return hipErrorInvalidConfiguration;
+  if (bytes <= lds_limit)
+    hipLaunchKernelGGL((last_solve_batched<TX, true>), dim3(nsys), dim3(1024), bytes, st, args);
+  else
+    hipLaunchKernelGGL((last_solve_batched<TX, false>), dim3(nsys), dim3(1024), xbytes, st, args);
+  return hipGetLastError();
+}
+
+template hipError_t launch_assemble_last_batched<float, double>(const AssembleArgs<float, double>*, int, hipStream_t);
+template hipError_t launch_assemble_last_batched<double, double>(const AssembleArgs<double, double>*, int, hipStream_t);
+template hipError_t launch_last_solve_batched<double>(const LastSolveArgs<double>*, int, int, int, size_t, hipStream_t);
 
 template <typename TW, typename TA>
 hipError_t launch_assemble_last(AssembleArgs<TW, TA> const& a, hipStream_t st) {

@@ -14,9 +14,9 @@ namespace bcr {
 
 
 template <typename TA>
-__global__ void __launch_bounds__(256) mv_rows_kernel(MvArgs<TA> a) {
+__device__ __forceinline__ void mv_rows_body(MvArgs<TA> a, int bid) {
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = bid * 4 + (threadIdx.x >> 6);
   const int N = a.N, n = a.n, nx = a.nx, nr = a.nr;
   if (k > N) return;
   const double* xk = a.x + size_t(k) * n;
@@ -42,7 +42,13 @@ __global__ void __launch_bounds__(256) mv_rows_kernel(MvArgs<TA> a) {
 }
 
 template <typename TA>
-__global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double* __restrict__ Csum) {
+__global__ void __launch_bounds__(256) mv_rows_kernel(MvArgs<TA> a) {
+  mv_rows_body<TA>(a, blockIdx.x);
+}
+
+
+template <typename TA>
+__device__ __forceinline__ void mv_tail_body(MvArgs<TA> const& a, const double* __restrict__ Csum) {
   const int N = a.N, n = a.n, qr = a.qr, qx = a.qx, nr = a.nr, nx = a.nx;
   const int m = n + qr;
   const int Nc = 2 * n + qx + nx;
@@ -73,6 +79,38 @@ __global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double
     a.out[row] = a.b ? a.b[row] - acc : acc;
   }
 }
+
+template <typename TA>
+__global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double* __restrict__ Csum) {
+  mv_tail_body<TA>(a, Csum);
+}
+
+template <typename TA>
+__global__ void __launch_bounds__(256) mv_rows_batched(BatchTable<MvArgs<TA>> t) {
+  const int e = t.entry[blockIdx.x];
+  mv_rows_body<TA>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
+// one workgroup per system; Csum[s] = the system's summed border partials
+template <typename TA>
+__global__ void __launch_bounds__(256) mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum) {
+  mv_tail_body<TA>(args[blockIdx.x], Csum[blockIdx.x]);
+}
+
+template <typename TA>
+hipError_t launch_mv_rows_batched(BatchTable<MvArgs<TA>> const& t, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((mv_rows_batched<TA>), dim3(nblocks), dim3(256), 0, st, t);
+  return hipGetLastError();
+}
+
+template <typename TA>
+hipError_t launch_mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum, int nsys, hipStream_t st) {
+  hipLaunchKernelGGL((mv_tail_batched<TA>), dim3(nsys), dim3(256), 0, st, args, Csum);
+  return hipGetLastError();
+}
+
+template hipError_t launch_mv_rows_batched<double>(BatchTable<MvArgs<double>> const&, int, hipStream_t);
+template hipError_t launch_mv_tail_batched<double>(const MvArgs<double>*, const double* const*, int, hipStream_t);
 
 // sum of squares of v[0..len) -> tmp[blockIdx] (pass 1) ; out = sqrt(sum) (pass 2)
 __global__ void __launch_bounds__(256) sumsq_partial_kernel(const double* __restrict__ v, int64_t len,

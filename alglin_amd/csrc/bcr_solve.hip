@@ -17,9 +17,9 @@ namespace bcr {
 
 
 template <typename TW, typename TX, int METHOD>
-__global__ void __launch_bounds__(128) forward_kernel(SolveArgs<TW, TX> a) {
+__device__ __forceinline__ void forward_body(SolveArgs<TW, TX> a, int bid) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int p = blockIdx.x;
+  const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
   pair_index(p, a.L, a.N, j, jp, jpp);
@@ -95,9 +95,15 @@ __global__ void __launch_bounds__(128) forward_kernel(SolveArgs<TW, TX> a) {
 }
 
 template <typename TW, typename TX, int METHOD>
-__global__ void __launch_bounds__(128) backward_kernel(SolveArgs<TW, TX> a) {
+__global__ void __launch_bounds__(128) forward_kernel(SolveArgs<TW, TX> a) {
+  forward_body<TW, TX, METHOD>(a, blockIdx.x);
+}
+
+
+template <typename TW, typename TX, int METHOD>
+__device__ __forceinline__ void backward_body(SolveArgs<TW, TX> a, int bid) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int p = blockIdx.x;
+  const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
   pair_index(p, a.L, a.N, j, jp, jpp);
@@ -173,6 +179,12 @@ __global__ void __launch_bounds__(128) backward_kernel(SolveArgs<TW, TX> a) {
   }
 }
 
+template <typename TW, typename TX, int METHOD>
+__global__ void __launch_bounds__(128) backward_kernel(SolveArgs<TW, TX> a) {
+  backward_body<TW, TX, METHOD>(a, blockIdx.x);
+}
+
+
 // part: [count][m] (column j of the partial sums), dst[i] += sum_j part[j][i],
 // dst index i maps to dst[(i % mrow) + (i / mrow) * ldd].
 template <typename TP, typename TD>
@@ -222,6 +234,38 @@ hipError_t launch_backward(int method, SolveArgs<TW, TX> const& a, hipStream_t s
   else hipLaunchKernelGGL((backward_kernel<TW, TX, kLU>), grid, block, bytes, st, a);
   return hipGetLastError();
 }
+
+template <typename TW, typename TX, int METHOD, bool FWD>
+__global__ void __launch_bounds__(128) sweep_generic_batched(BatchTable<SolveArgs<TW, TX>> t) {
+  const int e = t.entry[blockIdx.x];
+  if (FWD) forward_body<TW, TX, METHOD>(t.args[e], int(blockIdx.x) - t.first[e]);
+  else backward_body<TW, TX, METHOD>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
+// one workgroup per pair; n is the largest block size of the launch
+template <typename TW, typename TX>
+hipError_t launch_sweep_generic_batched(bool forward, int method, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks,
+                                        int n, int nx, hipStream_t st) {
+  const size_t nn = size_t(n) * n;
+  const size_t fbytes = align16(tsize(n) * sizeof(TW)) + size_t(2 * n) * kRhsChunk * sizeof(TX);
+  const size_t bbytes = align16((3 * nn + size_t(n) * nx) * sizeof(TW)) + (size_t(3 * n + nx) + n) * kRhsChunk * sizeof(TX);
+  dim3 grid(nblocks), block(n <= 32 ? 64 : 128);
+  if (forward) {
+    if (method == kLU) hipLaunchKernelGGL((sweep_generic_batched<TW, TX, kLU, true>), grid, block, fbytes, st, t);
+    else hipLaunchKernelGGL((sweep_generic_batched<TW, TX, kQR, true>), grid, block, fbytes, st, t);
+  } else {
+    if (method == kQRP) hipLaunchKernelGGL((sweep_generic_batched<TW, TX, kQRP, false>), grid, block, bbytes, st, t);
+    else hipLaunchKernelGGL((sweep_generic_batched<TW, TX, kLU, false>), grid, block, bbytes, st, t);
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_sweep_generic_batched<double, double>(bool, int, BatchTable<SolveArgs<double, double>> const&,
+                                                                 int, int, int, hipStream_t);
+template hipError_t launch_sweep_generic_batched<float, float>(bool, int, BatchTable<SolveArgs<float, float>> const&,
+                                                               int, int, int, hipStream_t);
+template hipError_t launch_sweep_generic_batched<float, double>(bool, int, BatchTable<SolveArgs<float, double>> const&,
+                                                                int, int, int, hipStream_t);
 
 // dst[(i%mrow) + (i/mrow)*ldd] += sum_{k=first}^{first+count-1} part[k*m + i]
 template <typename TP, typename TD>

@@ -43,11 +43,11 @@ __device__ __forceinline__ T group_sum(T v) {
 }
 
 template <typename TW, typename TX, int NMAX, int QB>
-__global__ void __launch_bounds__(256) forward_lu_fast_kernel(SolveArgs<TW, TX> a) {
+__device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bid) {
   constexpr int G = 2 * NMAX, PG = 64 / G;
   const int lane = threadIdx.x & 63;
   const int r = lane % G, base = lane - r;
-  const int p = (int(blockIdx.x) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
+  const int p = (int(bid) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
   const bool live = p < a.npairs;
   const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
   const int64_t ldx = a.ldx;
@@ -111,11 +111,17 @@ __global__ void __launch_bounds__(256) forward_lu_fast_kernel(SolveArgs<TW, TX> 
 }
 
 template <typename TW, typename TX, int NMAX, int QB>
-__global__ void __launch_bounds__(256) backward_lu_fast_kernel(SolveArgs<TW, TX> a) {
+__global__ void __launch_bounds__(256) forward_lu_fast_kernel(SolveArgs<TW, TX> a) {
+  forward_lu_fast_body<TW, TX, NMAX, QB>(a, blockIdx.x);
+}
+
+
+template <typename TW, typename TX, int NMAX, int QB>
+__device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int bid) {
   constexpr int G = 2 * NMAX, PG = 64 / G;
   const int lane = threadIdx.x & 63;
   const int r = lane % G, base = lane - r;
-  const int p = (int(blockIdx.x) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
+  const int p = (int(bid) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
   const bool live = p < a.npairs;
   const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
   const int64_t ldx = a.ldx;
@@ -195,6 +201,19 @@ __global__ void __launch_bounds__(256) backward_lu_fast_kernel(SolveArgs<TW, TX>
   }
 }
 
+template <typename TW, typename TX, int NMAX, int QB>
+__global__ void __launch_bounds__(256) backward_lu_fast_kernel(SolveArgs<TW, TX> a) {
+  backward_lu_fast_body<TW, TX, NMAX, QB>(a, blockIdx.x);
+}
+
+
+template <typename TW, typename TX, int NMAX, int QB, bool FWD>
+__global__ void __launch_bounds__(256) sweep_lu_fast_batched(BatchTable<SolveArgs<TW, TX>> t) {
+  const int e = t.entry[blockIdx.x];
+  if (FWD) forward_lu_fast_body<TW, TX, NMAX, QB>(t.args[e], int(blockIdx.x) - t.first[e]);
+  else backward_lu_fast_body<TW, TX, NMAX, QB>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
 template <typename TW, typename TX, int NMAX>
 hipError_t launch_sweep_nmax(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st) {
   constexpr int PG = 64 / (2 * NMAX);
@@ -221,6 +240,38 @@ bool launch_sweep_lu_fast(bool forward, SolveArgs<TW, TX> const& a, hipStream_t 
   return true;
 }
 
+// pairs per workgroup of the sweep kernels for block size bound n (<= 32)
+int sweep_lu_fast_pairs_per_block(int n) {
+  const int nmax = n <= 4 ? 4 : n <= 8 ? 8 : n <= 16 ? 16 : 32;
+  return 4 * (64 / (2 * nmax));
+}
+
+template <typename TW, typename TX>
+bool launch_sweep_lu_fast_batched(bool forward, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks, int n, int nrhs,
+                                  hipStream_t st, hipError_t* err) {
+  if (n < 1 || n > 32) return false;
+  auto go = [&](auto k1, auto k4) {
+    if (nrhs == 1) hipLaunchKernelGGL(k1, dim3(nblocks), dim3(256), 0, st, t);
+    else hipLaunchKernelGGL(k4, dim3(nblocks), dim3(256), 0, st, t);
+    *err = hipGetLastError();
+    return true;
+  };
+#define BCR_SW(NM)                                                                                                \
+  return forward ? go(sweep_lu_fast_batched<TW, TX, NM, 1, true>, sweep_lu_fast_batched<TW, TX, NM, 4, true>)     \
+                 : go(sweep_lu_fast_batched<TW, TX, NM, 1, false>, sweep_lu_fast_batched<TW, TX, NM, 4, false>);
+  if (n <= 4) { BCR_SW(4) }
+  if (n <= 8) { BCR_SW(8) }
+  if (n <= 16) { BCR_SW(16) }
+  BCR_SW(32)
+#undef BCR_SW
+}
+
+template bool launch_sweep_lu_fast_batched<double, double>(bool, BatchTable<SolveArgs<double, double>> const&, int, int,
+                                                           int, hipStream_t, hipError_t*);
+template bool launch_sweep_lu_fast_batched<float, float>(bool, BatchTable<SolveArgs<float, float>> const&, int, int,
+                                                         int, hipStream_t, hipError_t*);
+template bool launch_sweep_lu_fast_batched<float, double>(bool, BatchTable<SolveArgs<float, double>> const&, int, int,
+                                                          int, hipStream_t, hipError_t*);
 template bool launch_sweep_lu_fast<double, double>(bool, SolveArgs<double, double> const&, hipStream_t, hipError_t*);
 template bool launch_sweep_lu_fast<float, float>(bool, SolveArgs<float, float> const&, hipStream_t, hipError_t*);
 template bool launch_sweep_lu_fast<float, double>(bool, SolveArgs<float, double> const&, hipStream_t, hipError_t*);

@@ -10,6 +10,7 @@
 #include <type_traits>
 
 #include "BorderedCR.hh"
+#include "bcr_batch.hh"
 
 using alglin_amd::BorderedCR;
 
@@ -543,5 +544,110 @@ int BABD_free(BABD_intType id) {
   std::lock_guard<std::mutex> g(g_mu);
   return g_solvers.erase(id) ? 0 : 1;
 }
+
+}  // extern "C"
+
+// ---- batched systems ---------------------------------------------------------
+struct alglin_batch {
+  std::unique_ptr<bcr::BatchSolver> s;
+  std::string err{"no error"};
+  bcr::BatchReport rep;
+};
+
+namespace {
+template <typename Fn>
+int bguard(alglin_batch_t h, Fn&& fn) {
+  if (!h) return 1;
+  try {
+    return fn();
+  } catch (std::exception const& e) {
+    h->err = e.what();
+    return 1;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+alglin_batch_t alglin_batch_create(int mixed) {
+  try {
+    auto* h = new alglin_batch;
+    h->s.reset(new bcr::BatchSolver(mixed != 0));
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void alglin_batch_destroy(alglin_batch_t h) { delete h; }
+
+int alglin_batch_plan(alglin_batch_t h, int32_t nsys, int32_t const* dims) {
+  return bguard(h, [&] {
+    if (nsys <= 0 || !dims) throw std::invalid_argument("alglin_batch_plan: no systems");
+    std::vector<bcr::SysDims> d(nsys);
+    for (int32_t s = 0; s < nsys; ++s) {
+      int32_t const* v = dims + 7 * size_t(s);
+      d[s] = bcr::SysDims{v[0], v[1], v[2], v[3], v[4], v[5], v[6]};
+    }
+    h->s->plan(d);
+    return 0;
+  });
+}
+
+int64_t alglin_batch_size(alglin_batch_t h, int32_t kind) {
+  if (!h || kind < 0 || kind >= bcr::kbKinds) return -1;
+  return h->s->total(kind);
+}
+
+int64_t alglin_batch_offset(alglin_batch_t h, int32_t kind, int32_t s) {
+  if (!h || kind < 0 || kind >= bcr::kbKinds || s < 0) return -1;
+  return h->s->offset(kind, s);
+}
+
+int alglin_batch_bind_device(alglin_batch_t h, double const* D, double const* E, double const* B, double const* C,
+                             double const* H, double const* Cq, double const* F) {
+  return bguard(h, [&] {
+    h->s->bind_device(D, E, B, C, H, Cq, F);
+    return 0;
+  });
+}
+
+int alglin_batch_generate(alglin_batch_t h, uint64_t seed, double shift) {
+  return bguard(h, [&] {
+    h->s->generate(seed, shift);
+    return 0;
+  });
+}
+
+int alglin_batch_solve(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter, int use_graph) {
+  return bguard(h, [&] {
+    h->rep = h->s->solve(B, X, tol, max_iter, use_graph != 0);
+    return h->rep.failed == 0 ? 0 : 2;
+  });
+}
+
+int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual, int32_t* sweeps, double summary[4]) {
+  return bguard(h, [&] {
+    auto const& st = h->s->status();
+    auto const& rel = h->s->rel_residual();
+    auto const& sw = h->s->sweeps();
+    for (size_t i = 0; i < st.size(); ++i) {
+      if (status) status[i] = st[i];
+      if (rel_residual) rel_residual[i] = rel[i];
+      if (sweeps) sweeps[i] = sw[i];
+    }
+    if (summary) {
+      summary[0] = h->rep.sweeps;
+      summary[1] = h->rep.max_rel_residual;
+      summary[2] = h->rep.failed;
+      summary[3] = h->rep.ms_total;
+    }
+    return 0;
+  });
+}
+
+char const* alglin_batch_last_error(alglin_batch_t h) { return h ? h->err.c_str() : "null handle"; }
+
+void* alglin_batch_stream(alglin_batch_t h) { return h ? static_cast<void*>(h->s->stream()) : nullptr; }
 
 }  // extern "C"

@@ -235,3 +235,15 @@ def mv(sys, x: np.ndarray) -> np.ndarray:
         C = sys["C"].astype(np.float64)
         Y[(N + 1) * n + qr:] = (np.einsum("kij,kjr->ir", C, xblk) + sys["Cq"] @ xq + sys["F"] @ xb)
     return Y[:, 0] if squeeze else Y
+
+
+def uneven_batch_dims(nsys, seed=0, nmin=4, nmax=48, Nmin=16, Nmax=4096, method=0):
+    """Shapes of a batch of uneven systems: n uniform in {nmin..nmax}, N
+    log-uniform in [Nmin, Nmax] (rounded), qr = qx = n // 4, nr = nx = 2.
+    Rows (N, n, qr, qx, nr, nx, method)."""
+    rng = np.random.default_rng(seed)
+    n = rng.integers(nmin, nmax + 1, nsys)
+    N = np.rint(np.exp(rng.uniform(np.log(Nmin), np.log(Nmax), nsys))).astype(np.int64)
+    q = n // 4
+    return np.stack([N, n, q, q, np.full(nsys, 2), np.full(nsys, 2), np.full(nsys, method)], 1).astype(np.int32)
+

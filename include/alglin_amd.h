@@ -152,6 +152,39 @@ int alglin_device_synchronize(void);
 /* device name, compute units, HBM bytes */
 int alglin_device_info(char* name, int32_t len, int32_t* cus, int64_t* hbm_bytes);
 
+/* ---- batched systems (extension: no counterpart in the reference) ------------
+   Many independent BABD systems of uneven shape solved together: one launch
+   per reduction level and kernel class for all systems, the launch sequence
+   optionally captured once into a HIP graph and replayed.  Each system is the
+   single-system algorithm above (BorderedCR at num_parallel_block = 1) with
+   LU last block.  mixed=1: FP32 factors + FP64 refinement per system.
+
+   dims: 7 int32 per system: N n qr qx nr nx method (0 LU, 1 QR, 2 QRP for the
+   pair eliminations); qr + nr == qx + nx.  Packed device arrays: kind
+   0 = D and E, 1 = B, 2 = C, 3 = H, 4 = Cq, 5 = F, 6 = x (right-hand sides and
+   solutions); system s starts at alglin_batch_offset(h, kind, s). */
+typedef struct alglin_batch* alglin_batch_t;
+alglin_batch_t alglin_batch_create(int mixed);
+void alglin_batch_destroy(alglin_batch_t h);
+int alglin_batch_plan(alglin_batch_t h, int32_t nsys, int32_t const* dims);
+int64_t alglin_batch_size(alglin_batch_t h, int32_t kind);
+int64_t alglin_batch_offset(alglin_batch_t h, int32_t kind, int32_t s);
+int alglin_batch_bind_device(alglin_batch_t h, double const* D, double const* E, double const* B, double const* C,
+                             double const* H, double const* Cq, double const* F);
+/* fill the bound arrays with seeded Gaussian systems: D_k = shift I + G/sqrt(n),
+   E_k = G/sqrt(n), the other blocks G (counter-based generator on the GPU) */
+int alglin_batch_generate(alglin_batch_t h, uint64_t seed, double shift);
+/* X = A^{-1} B for every system (device pointers, kind 6 layout); returns 0
+   when every system reached ||b - A x||/||b|| <= tol, 2 when some did not
+   (see alglin_batch_report), 1 on error */
+int alglin_batch_solve(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter, int use_graph);
+/* per system: status (bit 0 zero pivot, bit 1 singular last block, bit 2 tol
+   not reached), relative residual, refinement sweeps; summary = {sweeps,
+   max residual, failed systems, wall ms of the last solve}; any may be null */
+int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual, int32_t* sweeps, double summary[4]);
+char const* alglin_batch_last_error(alglin_batch_t h);
+void* alglin_batch_stream(alglin_batch_t h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,108 @@
+"""Batched uneven systems on the GPU: every system of the batch must equal the
+single-system algorithm (the oracle's BorderedCR at num_parallel_block = 1,
+same pair method, LU last block) on the same inputs, FP64 within 1e-10
+relative on x; mixed precision must reach the requested residual per
+system; graph replays must equal the eager launches bit for bit."""
+import numpy as np
+import pytest
+
+from alglin_amd import systems
+from alglin_amd.bcr import AlglinError, Batch, DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["LU", "QR", "QRP"]
+
+
+def mixed_dims():
+    """Shapes covering every kernel class: n in 1..48, N from 1 (no
+    reduction level) to 300, borders 0..3, all three pair methods."""
+    rows = []
+    ns = [1, 2, 3, 4, 5, 7, 8, 9, 12, 16, 17, 24, 32, 33, 40, 48]
+    Ns = [1, 2, 3, 5, 16, 37, 100, 300]
+    k = 0
+    for n in ns:
+        for N in Ns[(k % 3)::3]:
+            q = n // 4
+            r = k % 4
+            rows.append((N, n, q, q, r, r, k % 3))
+            k += 1
+    return np.asarray(rows, np.int32)
+
+
+def random_rhs(bt, seed):
+    L = bt.size("x")
+    return np.random.default_rng(seed).standard_normal(L)
+
+
+def oracle_x(oracle_mod, sysd, method, b):
+    o = oracle_mod.OracleBorderedCR(np.float64)
+    o.allocate(sysd["N"], sysd["n"], sysd["qr"], sysd["qx"], sysd["nr"], sysd["nx"])
+    o.load(sysd)
+    o.select(METHODS[method], "LU")
+    assert o.factorize(), o.last_error()
+    ok, x = o.solve(b)
+    assert ok
+    return x
+
+
+def test_batch_fp64_matches_oracle_per_system(oracle_mod):
+    dims = mixed_dims()
+    print(f"case batch fp64 nsys={len(dims)}", flush=True)
+    bt = Batch(mixed=False)
+    bt.plan(dims)
+    bt.generate(seed=7, shift=4.0)
+    b = random_rhs(bt, 1)
+    B = DeviceArray.from_host(b)
+    X = DeviceArray(bt.size("x"))
+    rc = bt.solve(B, X, tol=1e-12, max_iter=3, graph=False)
+    rep = bt.report()
+    assert rc == 0, rep
+    x = X.to_host()
+    for s, row in enumerate(dims):
+        sysd = bt.system(s)
+        nrows, ncols = systems.dims(sysd)
+        o = bt.offset("x", s)
+        bs = b[o:o + nrows]
+        xo = oracle_x(oracle_mod, sysd, int(row[6]), bs)
+        xs = x[o:o + ncols]
+        err = np.abs(xs - xo[:ncols]).max() / max(1.0, np.abs(xo[:ncols]).max())
+        assert err < 1e-10, (s, tuple(row), err)
+        res = np.linalg.norm(systems.mv(sysd, xs) - bs) / np.linalg.norm(bs)
+        assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 1e-3 * res + 1e-17, (s, res)
+
+
+def test_batch_mixed_reaches_tolerance_and_graph_is_exact():
+    dims = mixed_dims()
+    dims[:, 6] = 0                                  # LU everywhere
+    print(f"case batch mixed nsys={len(dims)}", flush=True)
+    bt = Batch(mixed=True)
+    bt.plan(dims)
+    bt.generate(seed=11, shift=4.0)
+    b = random_rhs(bt, 2)
+    B = DeviceArray.from_host(b)
+    X0 = DeviceArray(bt.size("x"))
+    assert bt.solve(B, X0, tol=1e-12, max_iter=30, graph=False) == 0, bt.report()
+    rep0 = bt.report()
+    x0 = X0.to_host()
+    assert rep0["max_rel_residual"] <= 1e-12
+    assert rep0["sweeps"].max() >= 1                 # FP32 factors alone do not get there
+    for s in range(0, len(dims), 5):
+        sysd = bt.system(s)
+        nrows, ncols = systems.dims(sysd)
+        o = bt.offset("x", s)
+        res = np.linalg.norm(systems.mv(sysd, x0[o:o + ncols]) - b[o:o + nrows]) / np.linalg.norm(b[o:o + nrows])
+        assert res <= 2e-12, (s, res)
+    # graph replays, with eager work between them, give the eager result exactly
+    X1 = DeviceArray(bt.size("x"))
+    for rep in range(3):
+        assert bt.solve(B, X1, tol=1e-12, max_iter=30, graph=True) == 0
+        assert np.array_equal(X1.to_host(), x0), rep
+        assert bt.solve(B, X0, tol=1e-12, max_iter=30, graph=False) == 0
+        assert np.array_equal(X0.to_host(), x0)
+
+
+def test_batch_rejects_rectangular_last_block():
+    bt = Batch(mixed=False)
+    with pytest.raises(AlglinError):
+        bt.plan([(10, 4, 2, 1, 1, 1, 0)])
