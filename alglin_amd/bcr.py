@@ -63,6 +63,7 @@ def lib():
             "alglin_bcr_set_precision": (ci, [vp, cs, ctypes.c_double, i32]),
             "alglin_bcr_refine_report": (ci, [vp, vp]),
             "alglin_bcr_set_fallback": (ci, [vp, ctypes.c_int]),
+            "alglin_bcr_set_pair_kernel": (ci, [vp, ctypes.c_int]),
             "alglin_bcr_timing": (ci, [vp, vp, ci]),
             "alglin_dev_malloc": (vp, [i64]),
             "alglin_dev_free": (ci, [vp]),
@@ -81,6 +82,22 @@ def lib():
             "alglin_batch_report": (ci, [vp, vp, vp, vp, vp]),
             "alglin_batch_last_error": (cs, [vp]),
             "alglin_batch_stream": (vp, [vp]),
+            "alglin_batch_generate_singular": (ci, [vp, ctypes.c_uint64, ctypes.c_double, i32, vp, ctypes.c_double]),
+            "alglin_batch_set_methods": (ci, [vp, i32, vp]),
+            "alglin_batch_solve_selected": (ci, [vp, vp, vp, ctypes.c_double, i32, vp, ctypes.c_float,
+                                                 ctypes.c_int]),
+            "alglin_batch_selection": (ci, [vp, vp, vp, vp]),
+            "alglin_mlp_num_params": (i32, []),
+            "alglin_mlp_create": (vp, [ctypes.c_uint64]),
+            "alglin_mlp_destroy": (None, [vp]),
+            "alglin_mlp_last_error": (cs, [vp]),
+            "alglin_mlp_get_params": (ci, [vp, vp, vp, vp]),
+            "alglin_mlp_set_params": (ci, [vp, vp, vp, vp]),
+            "alglin_mlp_load": (ci, [vp, cs]),
+            "alglin_mlp_save": (ci, [vp, cs]),
+            "alglin_mlp_loss_grad": (ci, [vp, vp, vp, i32, vp, vp]),
+            "alglin_mlp_train": (ci, [vp, vp, vp, i32, i32, ctypes.c_float, vp]),
+            "alglin_mlp_predict": (ci, [vp, vp, i32, vp]),
             "alglin_device_synchronize": (ci, []),
             "alglin_device_info": (ci, [ctypes.c_char_p, i32, vp, vp]),
             "BABD_factorize": (ci, [i32] * 7 + [vp, i32] * 4),
@@ -251,6 +268,10 @@ class BorderedCR:
     def select_precision(self, mode="FP64", tol=1e-12, max_iter=30):
         """'FP64' (reference arithmetic) or 'MIXED' (FP32 factors + FP64 refinement)."""
         self._check(lib().alglin_bcr_set_precision(self._h, mode.encode(), tol, max_iter), "select_precision")
+
+    def set_pair_kernel(self, variant: int):
+        """LU pair kernel for 16 < n <= 32: 0 panel wave + columns wave, 1 both waves factor the panel."""
+        self._check(lib().alglin_bcr_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def set_fallback(self, on: bool):
         self._check(lib().alglin_bcr_set_fallback(self._h, int(on)), "set_fallback")
@@ -432,14 +453,54 @@ class Batch:
         ptr = [ctypes.c_void_p(self.ops[k].data_ptr()) for k in ("D", "E", "B", "C", "H", "Cq", "F")]
         self._check(lib().alglin_batch_bind_device(self._h, *ptr), "bind_device")
 
+    def share_operator(self, other: "Batch"):
+        """Use another batch's device operator (same plan) instead of our own copy."""
+        if not np.array_equal(other.dims[:, :6], self.dims[:, :6]):
+            raise AlglinError("share_operator: the two batches have different shapes")
+        for a in self.ops.values():
+            a.free()
+        self.ops = other.ops
+        self._shared_from = other
+        ptr = [ctypes.c_void_p(self.ops[k].data_ptr()) for k in ("D", "E", "B", "C", "H", "Cq", "F")]
+        self._check(lib().alglin_batch_bind_device(self._h, *ptr), "bind_device")
+
     def size(self, kind: str) -> int:
         return int(lib().alglin_batch_size(self._h, self.KINDS.index(kind)))
 
     def offset(self, kind: str, s: int) -> int:
         return int(lib().alglin_batch_offset(self._h, self.KINDS.index(kind), s))
 
-    def generate(self, seed: int, shift: float):
-        self._check(lib().alglin_batch_generate(self._h, ctypes.c_uint64(seed), shift), "generate")
+    def generate(self, seed: int, shift: float, sing_k=None, eps: float = 0.0):
+        """Seeded Gaussian systems on the GPU; sing_k[s] >= 1 makes the pair
+        column [E_{k-1}; D_k] of system s rank deficient up to eps."""
+        if sing_k is None:
+            self._check(lib().alglin_batch_generate(self._h, ctypes.c_uint64(seed), shift), "generate")
+        else:
+            k = np.ascontiguousarray(sing_k, np.int32)
+            self._check(lib().alglin_batch_generate_singular(self._h, ctypes.c_uint64(seed), shift, len(k), _p(k),
+                                                             eps), "generate")
+
+    def set_methods(self, methods):
+        m = np.ascontiguousarray(methods, np.int32)
+        self.dims[:, 6] = m
+        self._check(lib().alglin_batch_set_methods(self._h, len(m), _p(m)), "set_methods")
+
+    def solve_selected(self, B, X, tol=1e-12, max_iter=30, selector=None, threshold=0.5, fallback=True) -> int:
+        """Mixed solve with per-system LU/QRP choice (see alglin_batch_solve_selected)."""
+        rc = lib().alglin_batch_solve_selected(self._h, ctypes.c_void_p(B.data_ptr()), ctypes.c_void_p(X.data_ptr()),
+                                               tol, max_iter, selector._h if selector is not None else None,
+                                               threshold, int(fallback))
+        if rc == 1:
+            raise AlglinError(f"batch solve_selected: {self.last_error()}")
+        return rc
+
+    def selection(self) -> dict:
+        ns = len(self.dims)
+        f = np.zeros((ns, 12), np.float32)
+        q = np.zeros(ns, np.int32)
+        b = np.zeros(ns, np.int32)
+        self._check(lib().alglin_batch_selection(self._h, _p(f), _p(q), _p(b)), "selection")
+        return dict(features=f, chose_qrp=q.astype(bool), fell_back=b.astype(bool))
 
     def load_system(self, s: int, sysd: dict):
         """Upload one system's blocks (system dict) into its packed slices."""
@@ -487,4 +548,74 @@ class Batch:
 
     def stream(self) -> int:
         return lib().alglin_batch_stream(self._h)
+
+
+class Mlp:
+    """The GPU-trained LU-vs-QRP selector (alglin_mlp_* in alglin_amd.h)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = lib().alglin_mlp_create(ctypes.c_uint64(seed))
+        if not self._h:
+            raise AlglinError("alglin_mlp_create failed")
+        self.nparams = int(lib().alglin_mlp_num_params())
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().alglin_mlp_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise AlglinError(f"{what}: {lib().alglin_mlp_last_error(self._h).decode()}")
+
+    def params(self):
+        p = np.zeros(self.nparams, np.float32)
+        mu = np.zeros(12, np.float32)
+        sig = np.zeros(12, np.float32)
+        self._check(lib().alglin_mlp_get_params(self._h, _p(p), _p(mu), _p(sig)), "get_params")
+        return p, mu, sig
+
+    def set_params(self, p=None, mu=None, sig=None):
+        pp = None if p is None else np.ascontiguousarray(p, np.float32)
+        m = None if mu is None else np.ascontiguousarray(mu, np.float32)
+        sg = None if sig is None else np.ascontiguousarray(sig, np.float32)
+        self._check(lib().alglin_mlp_set_params(self._h, None if pp is None else _p(pp), None if m is None else _p(m),
+                                                None if sg is None else _p(sg)), "set_params")
+
+    def load(self, path):
+        self._check(lib().alglin_mlp_load(self._h, str(path).encode()), "load")
+
+    def save(self, path):
+        self._check(lib().alglin_mlp_save(self._h, str(path).encode()), "save")
+
+    @staticmethod
+    def _dev(X, y=None):
+        Xd = DeviceArray.from_host(np.ascontiguousarray(X, np.float32))
+        yd = None if y is None else DeviceArray.from_host(np.ascontiguousarray(y, np.int32))
+        return Xd, yd
+
+    def loss_grad(self, X, y):
+        Xd, yd = self._dev(X, y)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        self._check(lib().alglin_mlp_loss_grad(self._h, ctypes.c_void_p(Xd.data_ptr()), ctypes.c_void_p(yd.data_ptr()),
+                                               len(y), _p(loss), _p(g)), "loss_grad")
+        return float(loss[0]), g
+
+    def train(self, X, y, epochs=300, lr=3e-3):
+        Xd, yd = self._dev(X, y)
+        hist = np.zeros(epochs, np.float32)
+        self._check(lib().alglin_mlp_train(self._h, ctypes.c_void_p(Xd.data_ptr()), ctypes.c_void_p(yd.data_ptr()),
+                                           len(y), epochs, lr, _p(hist)), "train")
+        return hist
+
+    def predict(self, X):
+        Xd, _ = self._dev(X)
+        out = DeviceArray(len(X), np.float32)
+        self._check(lib().alglin_mlp_predict(self._h, ctypes.c_void_p(Xd.data_ptr()), len(X),
+                                             ctypes.c_void_p(out.data_ptr())), "predict")
+        return out.to_host()
 

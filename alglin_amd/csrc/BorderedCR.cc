@@ -223,10 +223,6 @@ bool BorderedCR<T>::factorize() {
   factorized_ = false;
   try {
     if (debug_) check_matrix();
-    if (int(last_) > int(LastChoice::QRP)) {
-      last_error_ = std::string("BorderedCR::factorize, ") + kLast[int(last_)] + " is not available on the GPU path";
-      return false;
-    }
     int32_t st = 0;
     device_bound_ = false;   // the host blocks become the operator
     if (mixed_) {
@@ -234,6 +230,7 @@ bool BorderedCR<T>::factorize() {
         auto& e = mix_->engine();
         e.set_method(int(sel_));
         e.set_last_method(int(last_));
+        e.set_last2(last2_ != LastChoice2::NONE);
         e.upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
         e.factorize_async();
         st = e.status_sync();
@@ -241,6 +238,7 @@ bool BorderedCR<T>::factorize() {
     } else {
       eng_->set_method(int(sel_));
       eng_->set_last_method(int(last_));
+      eng_->set_last2(last2_ != LastChoice2::NONE);
       eng_->upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
       eng_->factorize_async();
       st = eng_->status_sync();
@@ -328,23 +326,24 @@ void BorderedCR<T>::bind_device(T const* D, T const* E, T const* B, T const* C, 
 
 template <typename T>
 int32_t BorderedCR<T>::factorize_device(bool sync) {
-  if (int(last_) > int(LastChoice::QRP))
-    throw std::runtime_error(std::string("factorize_device: ") + kLast[int(last_)] + " is not available on the GPU path");
   int32_t st = 0;
   if (mixed_) {
     if constexpr (std::is_same_v<T, double>) {
       auto& e = mix_->engine();
       e.set_method(int(sel_));
       e.set_last_method(int(last_));
+      e.set_last2(last2_ != LastChoice2::NONE);
       e.factorize_async();
       if (sync) st = e.status_sync();
     }
   } else {
     eng_->set_method(int(sel_));
     eng_->set_last_method(int(last_));
+    eng_->set_last2(last2_ != LastChoice2::NONE);
     eng_->factorize_async();
     if (sync) st = eng_->status_sync();
   }
+  st &= 3;   // bit 3 only records that the second last-block solver was used
   factorized_ = st == 0;
   return st;
 }
@@ -371,6 +370,8 @@ void BorderedCR<T>::fallback_qrp(T* X, integer nrhs, int64_t ld) {
   if constexpr (std::is_same_v<T, double>) {
     if (!device_bound_) eng_->upload(D_.data(), E_.data(), B_.data(), C_.data(), H_.data(), Cq_.data(), F_.data());
     eng_->set_method(int(Choice::QRP));
+    eng_->set_last_method(int(last_));
+    eng_->set_last2(last2_ != LastChoice2::NONE);
     eng_->factorize_async();
     const int32_t st = eng_->status_sync();
     eng_->set_method(int(sel_));

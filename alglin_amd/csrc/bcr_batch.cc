@@ -1,6 +1,8 @@
 // Batched cyclic reduction (see bcr_batch.hh).
 #include "bcr_batch.hh"
 
+#include "bcr_mlp.hh"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -139,23 +141,52 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
   Csum_.resize(one(offCsum_[ns]));
   factor_bytes_ = sizeof(TW) * (2 * tot_[kbDE] + tot_[kbB] + tot_[kbC] + offT_[ns] + offCpend_[ns] + offFpart_[ns]) +
                   4 * offPerm_[ns] + 8 * offHm_[ns];
+  active_.resize(ns);
+  for (int s = 0; s < ns; ++s) active_[s] = s;
+  invalidate();
+  pD_ = pE_ = pB_ = pC_ = pH_ = pCq_ = pF_ = nullptr;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::invalidate() {
   tables_ok_ = false;
   solve_cache_.clear();
   mv_cache_.clear();
   norm_cache_.clear();
-  pD_ = pE_ = pB_ = pC_ = pH_ = pCq_ = pF_ = nullptr;
+  add_cache_.clear();
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::set_active(std::vector<char> const& mask) {
+  active_.clear();
+  for (int s = 0; s < nsys(); ++s)
+    if (s < int(mask.size()) && mask[s]) active_.push_back(s);
+  invalidate();
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::set_method(int s, int method) {
+  if (s < 0 || s >= nsys()) throw HipError("set_method: bad system index");
+  if (method != kLU && method != kQR && method != kQRP) throw HipError("set_method: method must be LU, QR or QRP");
+  SysDims d = dims_[s];
+  d.method = method;
+  if (factor_class(d) >= 4 && size_t(2 * d.n + d.nr) * (3 * d.n + d.nx) * sizeof(TW) > lds_limit_)
+    throw HipError("set_method: pair panel does not fit in LDS");
+  if (dims_[s].method != method) {
+    dims_[s].method = method;
+    invalidate();
+  }
 }
 
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::bind_device(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq,
                                       TA const* F) {
   pD_ = D; pE_ = E; pB_ = B; pC_ = C; pH_ = H; pCq_ = Cq; pF_ = F;
-  tables_ok_ = false;
-  mv_cache_.clear();
+  invalidate();
 }
 
 template <typename TA, typename TW>
-void BatchEngine<TA, TW>::generate(uint64_t seed, double shift) {
+void BatchEngine<TA, TW>::generate(uint64_t seed, double shift, std::vector<int32_t> const& sing_k, double eps) {
   if constexpr (std::is_same_v<TA, double>) {
     if (!pD_) throw HipError("batch generate: no operator bound");
     std::vector<GenArgs> g(nsys());
@@ -163,6 +194,8 @@ void BatchEngine<TA, TW>::generate(uint64_t seed, double shift) {
       SysDims const& d = dims_[s];
       GenArgs& a = g[s];
       a.sys = s; a.N = d.N; a.n = d.n; a.qr = d.qr; a.qx = d.qx; a.nr = d.nr; a.nx = d.nx; a.shift = shift;
+      a.sing_k = s < int(sing_k.size()) && sing_k[s] >= 1 && sing_k[s] < d.N ? sing_k[s] : -1;
+      a.eps = eps;
       a.D = const_cast<double*>(pD_) + off_[kbDE][s];
       a.E = const_cast<double*>(pE_) + off_[kbDE][s];
       a.B = const_cast<double*>(pB_) + off_[kbB][s];
@@ -199,7 +232,7 @@ void BatchEngine<TA, TW>::build_factor_tables() {
       std::vector<int32_t> entry, first{0};
       int nmax = 0, nxmax = 0, nrmax = 0;
       std::vector<int> members;
-      for (int s = 0; s < ns; ++s) {
+      for (int s : active_) {
         SysDims const& d = dims_[s];
         if (d.N <= L || factor_class(d) != cls) continue;
         members.push_back(s);
@@ -251,32 +284,34 @@ void BatchEngine<TA, TW>::build_factor_tables() {
       pend[lev].push_back(p);
     }
   }
-  // per-system records
-  std::vector<PrepArgs<TA, TW>> prep(ns);
-  std::vector<ReduceArgs<TW, double>> fred(ns);
-  std::vector<AssembleArgs<TW, TA>> as(ns);
-  std::vector<LastSolveArgs<double>> lu(ns);
-  std::vector<int32_t*> stp(ns);
-  for (int s = 0; s < ns; ++s) {
+  // per-system records (active systems, in order)
+  const int na = int(active_.size());
+  std::vector<PrepArgs<TA, TW>> prep(na);
+  std::vector<ReduceArgs<TW, double>> fred(na);
+  std::vector<AssembleArgs<TW, TA>> as(na);
+  std::vector<LastSolveArgs<double>> lu(na);
+  std::vector<int32_t*> stp(na);
+  for (int i = 0; i < na; ++i) {
+    const int s = active_[i];
     SysDims const& d = dims_[s];
-    PrepArgs<TA, TW>& p = prep[s];
+    PrepArgs<TA, TW>& p = prep[i];
     p.N = d.N; p.n = d.n; p.nr = d.nr; p.nx = d.nx;
     p.D = pD_ + off_[kbDE][s]; p.E = pE_ + off_[kbDE][s]; p.B = pB_ + off_[kbB][s]; p.C = pC_ + off_[kbC][s];
     p.wD = wD_.p + off_[kbDE][s]; p.wE = wE_.p + off_[kbDE][s]; p.wB = wB_.p + off_[kbB][s];
     p.wC = wC_.p + off_[kbC][s];
-    ReduceArgs<TW, double>& f = fred[s];
+    ReduceArgs<TW, double>& f = fred[i];
     f.part = Fpart_.p + offFpart_[s]; f.first = 1; f.count = d.N - 1; f.m = d.nr * d.nx; f.mrow = std::max(1, f.m);
     f.ldd = f.mrow; f.dst = Feff_.p + off_[kbF][s];
-    AssembleArgs<TW, TA>& a = as[s];
+    AssembleArgs<TW, TA>& a = as[i];
     a.N = d.N; a.n = d.n; a.qr = d.qr; a.qx = d.qx; a.nr = d.nr; a.nx = d.nx; a.Nr = d.Nr(); a.Nc = d.Nc();
     a.D0 = wD_.p + off_[kbDE][s]; a.E0 = wE_.p + off_[kbDE][s]; a.B0 = wB_.p + off_[kbB][s];
     a.C = wC_.p + off_[kbC][s]; a.Cpend = Cpend_.p + offCpend_[s];
     a.H = pH_ + off_[kbH][s]; a.Cq = pCq_ + off_[kbCq][s]; a.Feff = Feff_.p + off_[kbF][s];
     a.Hm = Hm_.p + offHm_[s];
-    LastSolveArgs<double>& l = lu[s];
+    LastSolveArgs<double>& l = lu[i];
     l = LastSolveArgs<double>{};
     l.Nc = d.Nc(); l.Nr = d.Nr(); l.Hm = Hm_.p + offHm_[s]; l.P = ipiv_.p + offPiv_[s];
-    stp[s] = status_.p + s;
+    stp[i] = status_.p + s;
   }
   const size_t oprep = bb.add(prep), ofred = bb.add(fred), oasm = bb.add(as), olu = bb.add(lu), ost = bb.add(stp);
   bb.upload(fblob_, st_);
@@ -301,8 +336,9 @@ void BatchEngine<TA, TW>::build_factor_tables() {
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::factorize_async() {
   if (!tables_ok_) build_factor_tables();
-  const int ns = nsys();
-  hip_check(launch_convert<int32_t, int32_t>(nullptr, status_.p, ns, st_), "status");
+  const int ns = int(active_.size());
+  if (ns == 0) return;
+  hip_check(launch_convert<int32_t, int32_t>(nullptr, status_.p, nsys(), st_), "status");
   hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
   hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
   for (int k = 0; k < levels_; ++k)
@@ -313,7 +349,7 @@ void BatchEngine<TA, TW>::factorize_async() {
         BatchTable<A> t{args, l.entry, l.first};
         bool ok = true;
         if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
-        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
+        else if (l.cls == 3) ok = launch_pair_lu_tw_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
         else {
           const size_t bytes = size_t(2 * l.nmax + l.nrmax) * (3 * l.nmax + l.nxmax) * sizeof(TW);
           e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, st_);
@@ -352,7 +388,7 @@ typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(dou
       std::vector<SolveArgs<TW, double>> args;
       std::vector<int32_t> entry, first{0};
       int nmax = 0, nxmax = 0;
-      for (int s = 0; s < ns; ++s) {
+      for (int s : active_) {
         SysDims const& d = dims_[s];
         if (d.N <= L || sweep_class(d) != cls) continue;
         SolveArgs<TW, double> a{};
@@ -384,15 +420,17 @@ typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(dou
       pend[lev].push_back(p);
     }
   }
-  std::vector<ReduceArgs<double, double>> xb(ns);
-  std::vector<LastSolveArgs<double>> last(ns);
-  for (int s = 0; s < ns; ++s) {
+  const int na = int(active_.size());
+  std::vector<ReduceArgs<double, double>> xb(na);
+  std::vector<LastSolveArgs<double>> last(na);
+  for (int i = 0; i < na; ++i) {
+    const int s = active_[i];
     SysDims const& d = dims_[s];
-    ReduceArgs<double, double>& r = xb[s];
+    ReduceArgs<double, double>& r = xb[i];
     r.part = Xpart_.p + offXpart_[s]; r.first = 1; r.count = d.N - 1; r.m = d.nr; r.mrow = std::max(1, d.nr);
     r.ldd = std::max(d.nrows(), d.ncols());
     r.dst = X + off_[kbX][s] + int64_t(d.N + 1) * d.n + d.qr;
-    LastSolveArgs<double>& l = last[s];
+    LastSolveArgs<double>& l = last[i];
     l.N = d.N; l.n = d.n; l.qr = d.qr; l.qx = d.qx; l.nr = d.nr; l.nx = d.nx; l.Nr = d.Nr(); l.Nc = d.Nc();
     l.method = kLastLU; l.nrhs = 1; l.Hm = Hm_.p + offHm_[s]; l.P = ipiv_.p + offPiv_[s]; l.Q = nullptr;
     l.tau = nullptr; l.x = X + off_[kbX][s]; l.ldx = std::max(d.nrows(), d.ncols());
@@ -417,7 +455,8 @@ typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(dou
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::solve_async(double* X) {
   SolveTables& T = solve_tables(X);
-  const int ns = nsys();
+  const int ns = int(active_.size());
+  if (ns == 0) return;
   auto sweep = [&](Launch const& l, bool fwd) {
     BatchTable<SolveArgs<TW, double>> t{static_cast<const SolveArgs<TW, double>*>(l.args), l.entry, l.first};
     hipError_t e = hipSuccess;
@@ -444,26 +483,27 @@ typename BatchEngine<TA, TW>::MvTables& BatchEngine<TA, TW>::mv_tables(double co
   if (it != mv_cache_.end()) return it->second;
   if (!pD_) throw HipError("batch residual: no operator bound");
   MvTables& T = mv_cache_[key];
-  const int ns = nsys();
-  std::vector<MvArgs<TA>> mv(ns);
+  const int na = int(active_.size());
+  std::vector<MvArgs<TA>> mv(na);
   std::vector<int32_t> entry, first{0};
-  std::vector<ReduceArgs<double, double>> cs(ns);
-  std::vector<const double*> csp(ns);
-  for (int s = 0; s < ns; ++s) {
+  std::vector<ReduceArgs<double, double>> cs(na);
+  std::vector<const double*> csp(na);
+  for (int i = 0; i < na; ++i) {
+    const int s = active_[i];
     SysDims const& d = dims_[s];
-    MvArgs<TA>& m = mv[s];
+    MvArgs<TA>& m = mv[i];
     m.N = d.N; m.n = d.n; m.qr = d.qr; m.qx = d.qx; m.nr = d.nr; m.nx = d.nx;
     m.D = pD_ + off_[kbDE][s]; m.E = pE_ + off_[kbDE][s]; m.B = pB_ + off_[kbB][s]; m.C = pC_ + off_[kbC][s];
     m.H = pH_ + off_[kbH][s]; m.Cq = pCq_ + off_[kbCq][s]; m.F = pF_ + off_[kbF][s];
     m.x = X + off_[kbX][s]; m.b = B ? B + off_[kbX][s] : nullptr; m.out = R + off_[kbX][s];
     m.Cpart = Cpart_.p + offCpart_[s];
     const int blocks = (d.N + 1 + 3) / 4;
-    for (int b = 0; b < blocks; ++b) entry.push_back(s);
+    for (int b = 0; b < blocks; ++b) entry.push_back(i);
     first.push_back(first.back() + blocks);
-    ReduceArgs<double, double>& r = cs[s];
+    ReduceArgs<double, double>& r = cs[i];
     r.part = Cpart_.p + offCpart_[s]; r.first = 0; r.count = d.N + 1; r.m = d.nr; r.mrow = std::max(1, d.nr);
     r.ldd = r.mrow; r.dst = Csum_.p + offCsum_[s];
-    csp[s] = Csum_.p + offCsum_[s];
+    csp[i] = Csum_.p + offCsum_[s];
   }
   Blob bb;
   const size_t om = bb.add(mv), oe = bb.add(entry), of = bb.add(first), oc = bb.add(cs), op = bb.add(csp);
@@ -482,12 +522,39 @@ typename BatchEngine<TA, TW>::MvTables& BatchEngine<TA, TW>::mv_tables(double co
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::residual_async(double const* X, double const* B, double* R) {
   MvTables& T = mv_tables(X, B, R);
-  const int ns = nsys();
+  const int ns = int(active_.size());
+  if (ns == 0) return;
   BatchTable<MvArgs<TA>> t{static_cast<const MvArgs<TA>*>(T.rows.args), T.rows.entry, T.rows.first};
   hip_check(launch_mv_rows_batched<TA>(t, T.rows.nblocks, st_), "batch mv rows");
   hip_check(launch_convert<double, double>(nullptr, Csum_.p, int64_t(Csum_.count), st_), "zero Csum");
   hip_check(launch_reduce_batched<double, double>(T.csum, ns, st_), "reduce Csum");
   hip_check(launch_mv_tail_batched<TA>(T.tail, T.csum_ptr, ns, st_), "batch mv tail");
+}
+
+template <typename TA, typename TW>
+const CopyArgs* BatchEngine<TA, TW>::add_table(double const* src, double* dst) {
+  std::vector<const void*> key{src, dst};
+  auto it = add_cache_.find(key);
+  if (it != add_cache_.end()) return it->second.args;
+  AddTables& T = add_cache_[key];
+  std::vector<CopyArgs> c(active_.size());
+  for (size_t i = 0; i < active_.size(); ++i) {
+    const int s = active_[i];
+    c[i].src = src + off_[kbX][s];
+    c[i].dst = dst + off_[kbX][s];
+    c[i].len = dims_[s].ncols();
+  }
+  Blob bb;
+  const size_t o = bb.add(c);
+  bb.upload(T.blob, st_);
+  T.args = reinterpret_cast<const CopyArgs*>(T.blob.p + o);
+  return T.args;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::add_active_async(double const* src, double* dst) {
+  const CopyArgs* t = add_table(src, dst);
+  if (!active_.empty()) hip_check(launch_add_batched(t, int(active_.size()), st_), "x += d");
 }
 
 template <typename TA, typename TW>
@@ -497,6 +564,7 @@ void BatchEngine<TA, TW>::prepare(double* X, double const* B, double* R) {
   solve_tables(R);
   mv_tables(X, B, R);
   norms_async(B, R, nullptr);
+  add_table(R, X);
 }
 
 template <typename TA, typename TW>
@@ -505,11 +573,13 @@ void BatchEngine<TA, TW>::norms_async(double const* B, double const* R, double* 
   auto it = norm_cache_.find(key);
   if (it == norm_cache_.end()) {
     NormTables& T = norm_cache_[key];
-    std::vector<NormArgs> a(nsys());
-    for (int s = 0; s < nsys(); ++s) {
-      a[s].b = B + off_[kbX][s];
-      a[s].r = R + off_[kbX][s];
-      a[s].len = dims_[s].nrows();
+    std::vector<NormArgs> a(active_.size());
+    for (size_t i = 0; i < active_.size(); ++i) {
+      const int s = active_[i];
+      a[i].b = B + off_[kbX][s];
+      a[i].r = R + off_[kbX][s];
+      a[i].len = dims_[s].nrows();
+      a[i].sys = s;
     }
     Blob bb;
     const size_t o = bb.add(a);
@@ -517,7 +587,45 @@ void BatchEngine<TA, TW>::norms_async(double const* B, double const* R, double* 
     T.args = reinterpret_cast<const NormArgs*>(T.blob.p + o);
     it = norm_cache_.find(key);
   }
-  if (out) hip_check(launch_norms_batched(it->second.args, nsys(), out, st_), "batch norms");
+  if (out && !active_.empty())
+    hip_check(launch_norms_batched(it->second.args, int(active_.size()), out, st_), "batch norms");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::features_async(double const* norms, float* out) {
+  if (!tables_ok_) build_factor_tables();
+  std::vector<FeatArgs<TW>> f(active_.size());
+  for (size_t i = 0; i < active_.size(); ++i) {
+    const int s = active_[i];
+    SysDims const& d = dims_[s];
+    FeatArgs<TW>& a = f[i];
+    a.N = d.N; a.n = d.n; a.qr = d.qr; a.nr = d.nr; a.Nc = d.Nc(); a.sys = s;
+    a.Tm = Tm_.p + offT_[s]; a.Hm = Hm_.p + offHm_[s]; a.norms = norms;
+  }
+  if (f.empty()) return;
+  Blob bb;
+  const size_t o = bb.add(f);
+  bb.upload(scratch_, st_);
+  hip_check(launch_features_batched<TW>(reinterpret_cast<const FeatArgs<TW>*>(scratch_.p + o), int(f.size()), out, st_),
+            "features");
+  hip_check(hipStreamSynchronize(st_), "features sync");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::copy_active_async(double const* src, double* dst) {
+  std::vector<CopyArgs> c(active_.size());
+  for (size_t i = 0; i < active_.size(); ++i) {
+    const int s = active_[i];
+    c[i].src = src + off_[kbX][s];
+    c[i].dst = dst + off_[kbX][s];
+    c[i].len = std::max(dims_[s].nrows(), dims_[s].ncols());
+  }
+  if (c.empty()) return;
+  Blob bb;
+  const size_t o = bb.add(c);
+  bb.upload(scratch_, st_);
+  hip_check(launch_copy_batched(reinterpret_cast<const CopyArgs*>(scratch_.p + o), int(c.size()), st_), "copy");
+  hip_check(hipStreamSynchronize(st_), "copy sync");
 }
 
 template class BatchEngine<double, double>;
@@ -546,8 +654,13 @@ hipStream_t BatchSolver::stream() const { return mixed_ ? e32_->stream() : e64_-
 
 void BatchSolver::plan(std::vector<SysDims> const& dims) {
   drop_graphs();
-  if (mixed_) e32_->plan(dims);
-  else e64_->plan(dims);
+  if (mixed_) {
+    e32_->plan(dims);
+    if (e64_) e64_->plan(dims);
+  } else {
+    e64_->plan(dims);
+  }
+  dims_ = dims;
   R_.resize(size_t(std::max<int64_t>(total(kbX), 1)));
   norms_.resize(2 * dims.size());
 }
@@ -558,13 +671,23 @@ int64_t BatchSolver::offset(int kind, int s) const { return mixed_ ? e32_->offse
 void BatchSolver::bind_device(double const* D, double const* E, double const* B, double const* C, double const* H,
                               double const* Cq, double const* F) {
   drop_graphs();
+  oD_ = D; oE_ = E; oB_ = B; oC_ = C; oH_ = H; oCq_ = Cq; oF_ = F;
   if (mixed_) e32_->bind_device(D, E, B, C, H, Cq, F);
-  else e64_->bind_device(D, E, B, C, H, Cq, F);
+  if (e64_) e64_->bind_device(D, E, B, C, H, Cq, F);
 }
 
-void BatchSolver::generate(uint64_t seed, double shift) {
-  if (mixed_) e32_->generate(seed, shift);
-  else e64_->generate(seed, shift);
+void BatchSolver::generate(uint64_t seed, double shift, std::vector<int32_t> const& sing_k, double eps) {
+  if (mixed_) e32_->generate(seed, shift, sing_k, eps);
+  else e64_->generate(seed, shift, sing_k, eps);
+}
+
+void BatchSolver::set_methods(std::vector<int32_t> const& m) {
+  drop_graphs();
+  for (size_t s = 0; s < m.size() && s < dims_.size(); ++s) {
+    dims_[s].method = m[s];
+    if (mixed_) e32_->set_method(int(s), m[s]);
+    else e64_->set_method(int(s), m[s]);
+  }
 }
 
 namespace {
@@ -587,54 +710,30 @@ hipGraphExec_t capture(hipStream_t st, F&& body) {
 }
 }  // namespace
 
+// Refinement sweeps over the systems not yet done: residual, host test
+// (converged / stalled: three sweeps without halving the best residual /
+// factorization failed), correction solve.  With graphs the residual and
+// correction launch lists are replayed (they cover the engine's active set).
 template <typename E>
-BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph) {
-  const auto t0 = std::chrono::steady_clock::now();
+void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done,
+                         BatchReport& rep) {
   const int ns = eng.nsys();
   hipStream_t st = eng.stream();
   const int64_t nx = eng.total(kbX);
-  auto factor_solve = [&] {
-    hip_check(hipMemcpyAsync(X, B, size_t(nx) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
-    eng.factorize_async();
-    eng.solve_async(X);
-  };
-  auto residual = [&] {
-    eng.residual_async(X, B, R_.p);
-    eng.norms_async(B, R_.p, norms_.p);
-  };
-  auto correct = [&] {
-    eng.solve_async(R_.p);
-    hip_check(launch_axpy(1.0, R_.p, X, nx, st), "x += d");
-  };
-  if (use_graph) {
-    if (g_.B != B || g_.X != X) {
-      drop_graphs();
-      eng.prepare(X, B, R_.p);   // no allocation inside a capture
-      g_.fs = capture(st, factor_solve);
-      g_.res = capture(st, residual);
-      g_.corr = capture(st, correct);
-      g_.B = B;
-      g_.X = X;
-    }
-    hip_check(hipGraphLaunch(g_.fs, st), "graph factor+solve");
-  } else {
-    factor_solve();
-  }
-  status_.assign(ns, 0);
-  eng.status(status_.data());
-  rel_.assign(ns, 0.0);
-  sweeps_.assign(ns, 0);
+  const bool graph = g_.res && g_.corr;
   std::vector<double> best(ns, INFINITY), h(2 * size_t(ns));
   std::vector<int> stagnant(ns, 0);
-  std::vector<char> done(ns, 0);
-  BatchReport rep;
   for (int it = 0;; ++it) {
-    if (use_graph) hip_check(hipGraphLaunch(g_.res, st), "graph residual");
-    else residual();
+    if (graph) {
+      hip_check(hipGraphLaunch(g_.res, st), "graph residual");
+    } else {
+      eng.residual_async(X, B, R_.p);
+      eng.norms_async(B, R_.p, norms_.p);
+    }
     hip_check(hipMemcpyAsync(h.data(), norms_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "norms");
     hip_check(hipStreamSynchronize(st), "norms sync");
     int open = 0;
-    for (int s = 0; s < ns; ++s) {
+    for (int s : eng.active()) {
       if (done[s]) continue;
       const double nb = h[2 * s] > 0 ? h[2 * s] : 1.0;
       const double rel = h[2 * s + 1] / nb;
@@ -649,11 +748,70 @@ BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int
       best[s] = std::min(best[s], rel);
       ++open;
     }
-    rep.sweeps = it;
+    rep.sweeps = std::max(rep.sweeps, it);
     if (open == 0 || it >= max_iter) break;
-    if (use_graph) hip_check(hipGraphLaunch(g_.corr, st), "graph correction");
-    else correct();
+    // once most of the swept systems are done, sweep only the open ones (the
+    // launch tables are rebuilt for the smaller set; graphs cover the old set)
+    if (!graph && 2 * open <= int(eng.active().size())) {
+      std::vector<char> mask(ns, 0);
+      for (int s : eng.active())
+        if (!done[s]) mask[s] = 1;
+      eng.set_active(mask);
+    }
+    if (graph) {
+      hip_check(hipGraphLaunch(g_.corr, st), "graph correction");
+    } else {
+      eng.solve_async(R_.p);
+      eng.add_active_async(R_.p, X);
+    }
   }
+}
+
+template <typename E>
+BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int ns = eng.nsys();
+  hipStream_t st = eng.stream();
+  const int64_t nx = eng.total(kbX);
+  if (int(eng.active().size()) != ns) {   // a previous sweep narrowed the set
+    drop_graphs();
+    eng.set_active(std::vector<char>(ns, 1));
+  }
+  auto factor_solve = [&] {
+    hip_check(hipMemcpyAsync(X, B, size_t(nx) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
+    eng.factorize_async();
+    eng.solve_async(X);
+  };
+  auto residual = [&] {
+    eng.residual_async(X, B, R_.p);
+    eng.norms_async(B, R_.p, norms_.p);
+  };
+  auto correct = [&] {
+    eng.solve_async(R_.p);
+    eng.add_active_async(R_.p, X);
+  };
+  if (use_graph) {
+    if (g_.B != B || g_.X != X || !g_.fs) {
+      drop_graphs();
+      eng.prepare(X, B, R_.p);   // no allocation inside a capture
+      g_.fs = capture(st, factor_solve);
+      g_.res = capture(st, residual);
+      g_.corr = capture(st, correct);
+      g_.B = B;
+      g_.X = X;
+    }
+    hip_check(hipGraphLaunch(g_.fs, st), "graph factor+solve");
+  } else {
+    drop_graphs();
+    factor_solve();
+  }
+  status_.assign(ns, 0);
+  eng.status(status_.data());
+  rel_.assign(ns, 0.0);
+  sweeps_.assign(ns, 0);
+  std::vector<char> done(ns, 0);
+  BatchReport rep;
+  refine(eng, B, X, tol, max_iter, done, rep);
   for (int s = 0; s < ns; ++s) {
     rep.max_rel_residual = std::max(rep.max_rel_residual, rel_[s]);
     if (status_[s] == 0 && !(rel_[s] <= tol)) status_[s] |= 4;
@@ -665,6 +823,106 @@ BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int
 
 BatchReport BatchSolver::solve(double const* B, double* X, double tol, int max_iter, bool use_graph) {
   return mixed_ ? run(*e32_, B, X, tol, max_iter, use_graph) : run(*e64_, B, X, tol, max_iter, use_graph);
+}
+
+BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, int max_iter, Mlp* selector,
+                                        float threshold, bool fallback) {
+  if (!mixed_) throw HipError("solve_selected: needs the mixed batch solver");
+  const auto t0 = std::chrono::steady_clock::now();
+  drop_graphs();
+  auto& eng = *e32_;
+  const int ns = eng.nsys();
+  hipStream_t st = eng.stream();
+  BatchReport rep;
+  status_.assign(ns, 0);
+  rel_.assign(ns, 0.0);
+  sweeps_.assign(ns, 0);
+  chose_qrp_.assign(ns, 0);
+  fell_back_.assign(ns, 0);
+  feat_.assign(size_t(ns) * 12, 0.f);
+  // 1. FP32 LU everywhere, first solve and its residual
+  for (int s = 0; s < ns; ++s) eng.set_method(s, kLU);
+  eng.set_active(std::vector<char>(ns, 1));
+  hip_check(hipMemcpyAsync(X, B, size_t(eng.total(kbX)) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
+  eng.factorize_async();
+  eng.solve_async(X);
+  eng.residual_async(X, B, R_.p);
+  eng.norms_async(B, R_.p, norms_.p);
+  eng.status(status_.data());
+  // 2. features (always: they are also the selector's training inputs), selector
+  dfeat_.resize(size_t(ns) * 12);
+  eng.features_async(norms_.p, dfeat_.p);
+  hip_check(hipMemcpy(feat_.data(), dfeat_.p, feat_.size() * sizeof(float), hipMemcpyDeviceToHost), "features");
+  if (selector) {
+    dprob_.resize(size_t(ns));
+    selector->predict(dfeat_.p, ns, dprob_.p);
+    std::vector<float> p(ns);
+    hip_check(hipMemcpy(p.data(), dprob_.p, p.size() * sizeof(float), hipMemcpyDeviceToHost), "probabilities");
+    std::vector<char> mask(ns, 0);
+    int nq = 0;
+    for (int s = 0; s < ns; ++s)
+      if (p[s] > threshold) {
+        mask[s] = 1;
+        chose_qrp_[s] = 1;
+        eng.set_method(s, kQRP);
+        ++nq;
+      }
+    if (nq > 0) {
+      eng.set_active(mask);
+      eng.copy_active_async(B, X);
+      eng.factorize_async();
+      eng.solve_async(X);
+      std::vector<int32_t> st2(ns, 0);
+      eng.status(st2.data());
+      for (int s = 0; s < ns; ++s)
+        if (mask[s]) status_[s] = st2[s];
+      eng.set_active(std::vector<char>(ns, 1));
+    }
+  }
+  // 3. FP64 refinement of every system with its factors
+  std::vector<char> done(ns, 0);
+  refine(eng, B, X, tol, max_iter, done, rep);
+  // 4. FP64 QRP for the systems that did not get there
+  std::vector<char> bad(ns, 0);
+  int nbad = 0;
+  for (int s = 0; s < ns; ++s)
+    if (status_[s] != 0 || !(rel_[s] <= tol)) { bad[s] = 1; ++nbad; }
+  if (fallback && nbad > 0) {
+    // FP64 QRP elimination, refined in FP64 (its factors lose digits on
+    // nearly singular pair columns; the refinement recovers them while the
+    // FP64 backward error stays below one)
+    if (!e64_) {
+      e64_ = new BatchEngine<double, double>();
+      e64_->plan(dims_);
+      e64_->bind_device(oD_, oE_, oB_, oC_, oH_, oCq_, oF_);
+    }
+    auto& f = *e64_;
+    for (int s = 0; s < ns; ++s) f.set_method(s, kQRP);
+    f.set_active(bad);
+    hip_check(hipStreamSynchronize(st), "sync before fallback");
+    f.copy_active_async(B, X);
+    f.factorize_async();
+    f.solve_async(X);
+    std::vector<int32_t> st3(ns, 0);
+    f.status(st3.data());
+    std::vector<char> done2(ns, 1);
+    for (int s = 0; s < ns; ++s)
+      if (bad[s]) {
+        fell_back_[s] = 1;
+        status_[s] = st3[s];
+        done2[s] = 0;
+      }
+    BatchReport r2;
+    refine(f, B, X, tol, max_iter, done2, r2);
+    rep.sweeps = std::max(rep.sweeps, r2.sweeps);
+  }
+  for (int s = 0; s < ns; ++s) {
+    rep.max_rel_residual = std::max(rep.max_rel_residual, rel_[s]);
+    if (status_[s] == 0 && !(rel_[s] <= tol)) status_[s] |= 4;
+    if (status_[s] != 0) ++rep.failed;
+  }
+  rep.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rep;
 }
 
 }  // namespace bcr

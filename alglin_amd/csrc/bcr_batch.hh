@@ -52,14 +52,22 @@ class BatchEngine {
   // storage; throws HipError on an unsupported shape
   void plan(std::vector<SysDims> const& dims);
   int nsys() const { return int(dims_.size()); }
+  // restrict factorize/solve/residual/norms to the systems with mask[s] != 0
+  // (the others' factors and vectors are left untouched)
+  void set_active(std::vector<char> const& mask);
+  std::vector<int> const& active() const { return active_; }
+  // pair elimination method of system s (kLU, kQR, kQRP)
+  void set_method(int s, int method);
   SysDims const& dims(int s) const { return dims_[s]; }
   int64_t total(int kind) const { return tot_[kind]; }
   int64_t offset(int kind, int s) const { return off_[kind][s]; }
 
   // the pristine operator (packed device arrays, kinds as above)
   void bind_device(TA const* D, TA const* E, TA const* B, TA const* C, TA const* H, TA const* Cq, TA const* F);
-  // fills the bound arrays with the synthetic Gaussian systems (needs TA = double)
-  void generate(uint64_t seed, double shift);
+  // fills the bound arrays with the synthetic Gaussian systems (needs TA =
+  // double); sing_k (per system, may be empty): pair column made rank
+  // deficient up to eps (k >= 1), or none (k < 1)
+  void generate(uint64_t seed, double shift, std::vector<int32_t> const& sing_k = {}, double eps = 0.0);
 
   void factorize_async();
   // per-system status after factorize: bit 0 zero pivot in a pair, bit 1 singular last block
@@ -70,6 +78,14 @@ class BatchEngine {
   void residual_async(double const* X, double const* B, double* R);
   // out[2s] = ||B_s||, out[2s+1] = ||R_s|| (device array of 2*nsys)
   void norms_async(double const* B, double const* R, double* out);
+
+  // selector features of the active systems after factorize + solve +
+  // norms (out: 12 floats per system, at out[12*s])
+  void features_async(double const* norms, float* out);
+  // dst_s = src_s (kind x slices) for the active systems
+  void copy_active_async(double const* src, double* dst);
+  // dst_s += src_s over the active systems' solution entries (ncols each)
+  void add_active_async(double const* src, double* dst);
 
   // builds every launch table these buffers need (no launch); a graph
   // capture of factorize/solve/residual must not allocate
@@ -103,12 +119,19 @@ class BatchEngine {
     const NormArgs* args = nullptr;
     DevBuf<char> blob;
   };
+  struct AddTables {
+    const CopyArgs* args = nullptr;
+    DevBuf<char> blob;
+  };
 
   void build_factor_tables();
+  const CopyArgs* add_table(double const* src, double* dst);
   SolveTables& solve_tables(double* X);
   MvTables& mv_tables(double const* X, double const* B, double* R);
 
+  void invalidate();
   std::vector<SysDims> dims_;
+  std::vector<int> active_;
   std::vector<int64_t> off_[kbKinds];
   int64_t tot_[kbKinds] = {0, 0, 0, 0, 0, 0, 0};
   // factor-side offsets
@@ -130,12 +153,13 @@ class BatchEngine {
   const AssembleArgs<TW, TA>* asm_ = nullptr;
   const LastSolveArgs<double>* lu_ = nullptr;
   int32_t* const* stptr_ = nullptr;
-  DevBuf<char> fblob_;
+  DevBuf<char> fblob_, scratch_;
   bool tables_ok_ = false;
 
   std::map<const void*, SolveTables> solve_cache_;
   std::map<std::vector<const void*>, MvTables> mv_cache_;
   std::map<std::vector<const void*>, NormTables> norm_cache_;
+  std::map<std::vector<const void*>, AddTables> add_cache_;
 };
 
 // The batched solver a caller sees: FP64 cyclic reduction, or FP32 factors
@@ -157,9 +181,22 @@ class BatchSolver {
   int64_t offset(int kind, int s) const;
   void bind_device(double const* D, double const* E, double const* B, double const* C, double const* H,
                    double const* Cq, double const* F);
-  void generate(uint64_t seed, double shift);
+  void generate(uint64_t seed, double shift, std::vector<int32_t> const& sing_k = {}, double eps = 0.0);
   // B, X: packed device vectors (kind x); X = A^{-1} B per system
   BatchReport solve(double const* B, double* X, double tol, int max_iter, bool use_graph);
+  // mixed solver with per-system method choice: every system starts with FP32
+  // LU; after the first solve the selector (if given) sends the systems it
+  // scores above `threshold` to FP32 QRP; all are refined in FP64; with
+  // `fallback`, systems that still miss tol are solved by FP64 QRP
+  BatchReport solve_selected(double const* B, double* X, double tol, int max_iter, class Mlp* selector,
+                             float threshold, bool fallback);
+  // features of the last solve_selected (12 per system, host) and the
+  // systems sent to QRP / to the FP64 fallback
+  std::vector<float> const& features() const { return feat_; }
+  std::vector<char> const& chose_qrp() const { return chose_qrp_; }
+  std::vector<char> const& fell_back() const { return fell_back_; }
+  // per-system methods for solve(): kLU / kQR / kQRP
+  void set_methods(std::vector<int32_t> const& m);
   std::vector<int32_t> const& status() const { return status_; }
   std::vector<double> const& rel_residual() const { return rel_; }
   std::vector<int32_t> const& sweeps() const { return sweeps_; }
@@ -174,6 +211,15 @@ class BatchSolver {
   DevBuf<double> R_, norms_;
   std::vector<int32_t> status_, sweeps_;
   std::vector<double> rel_;
+  std::vector<float> feat_;
+  std::vector<char> chose_qrp_, fell_back_;
+  DevBuf<float> dfeat_, dprob_;
+  std::vector<SysDims> dims_;
+  double const *oD_ = nullptr, *oE_ = nullptr, *oB_ = nullptr, *oC_ = nullptr, *oH_ = nullptr, *oCq_ = nullptr,
+               *oF_ = nullptr;
+  // refinement loop over the active systems of `eng` (host convergence test)
+  template <typename E>
+  void refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done, BatchReport& rep);
   // captured graphs: factor+solve, residual+norms, correction
   struct Graphs {
     const double* B = nullptr;

@@ -27,14 +27,41 @@ struct NormArgs {
   const double* b;
   const double* r;
   int64_t len;
+  int sys;   // output slot: out[2*sys], out[2*sys+1]
 };
 
-// synthetic system s: D_k = shift I + G/sqrt(n), E_k = G/sqrt(n), B, C, H, Cq, F = G
+// synthetic system s: D_k = shift I + G/sqrt(n), E_k = G/sqrt(n), B, C, H, Cq, F = G;
+// with sing_k >= 1 the pair column block [E_{k-1}; D_k] (k = sing_k) is made
+// rank deficient up to eps: its last column becomes the mean of its first two
+// plus eps G/sqrt(n)
 struct GenArgs {
-  int sys, N, n, qr, qx, nr, nx;
-  double shift;
+  int sys, N, n, qr, qx, nr, nx, sing_k;
+  double shift, eps;
   double *D, *E, *B, *C, *H, *Cq, *F;
 };
+
+// per-system features of the LU-vs-QRP selector (12 floats at out[12*sys]),
+// from the factored pair panels, the factored last block and the residual
+// of the first solve
+template <typename TW>
+struct FeatArgs {
+  int N, n, qr, nr, Nc, sys;
+  const TW* Tm;          // N panels of tsize(n)
+  const double* Hm;      // factored Nc x Nc last block
+  const double* norms;   // norms[2*sys] = ||b||, norms[2*sys+1] = ||r||
+};
+
+struct CopyArgs {
+  const double* src;
+  double* dst;
+  int64_t len;
+};
+
+template <typename TW>
+hipError_t launch_features_batched(const FeatArgs<TW>* args, int nsys, float* out, hipStream_t st);
+hipError_t launch_copy_batched(const CopyArgs* args, int nsys, hipStream_t st);
+// dst_s += src_s per record (the same CopyArgs records)
+hipError_t launch_add_batched(const CopyArgs* args, int nsys, hipStream_t st);
 
 template <typename TA, typename TW>
 hipError_t launch_prep_batched(const PrepArgs<TA, TW>* args, int nsys, hipStream_t st);
@@ -57,6 +84,9 @@ bool launch_pair_lu_fast_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nb
 template <typename TS, typename TW>
 bool launch_pair_lu_split_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                   hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
+bool launch_pair_lu_tw_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
+                               hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 hipError_t launch_pair_generic_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, size_t bytes,
                                        size_t lds_limit, hipStream_t st);

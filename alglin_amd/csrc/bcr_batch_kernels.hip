@@ -71,8 +71,8 @@ __global__ void __launch_bounds__(256) norms_batched(const NormArgs* args, doubl
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = sqrt(red[0][0]);
-    out[2 * blockIdx.x + 1] = sqrt(red[1][0]);
+    out[2 * a.sys] = sqrt(red[0][0]);
+    out[2 * a.sys + 1] = sqrt(red[1][0]);
   }
 }
 
@@ -109,17 +109,122 @@ __global__ void __launch_bounds__(256) gen_batched(const GenArgs* args, int kind
   const double sc = (kind <= 1) ? 1.0 / sqrt(double(n)) : 1.0;
   const uint64_t base = ((uint64_t(a.sys) << 4) | uint64_t(kind)) << 40;
   const int64_t nn = int64_t(n) * n;
+  // the block of this kind that belongs to the rank-deficient pair column
+  const int64_t sblk = a.sing_k < 1 ? -1 : kind == 0 ? a.sing_k : kind == 1 ? a.sing_k - 1 : -1;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += int64_t(gridDim.x) * blockDim.x) {
     double v = sc * gauss(seed, base | uint64_t(i));
-    if (kind == 0) {
-      const int64_t w = i % nn;
-      if (w % n == w / n) v += a.shift;
+    int64_t w = 0;
+    if (kind <= 1) w = i % nn;
+    if (kind == 0 && w % n == w / n) v += a.shift;
+    if (kind <= 1 && n >= 3 && i / nn == sblk && w / n == n - 1) {
+      const int64_t r = w % n, b0 = (i / nn) * nn;
+      double c0 = sc * gauss(seed, base | uint64_t(b0 + r));
+      double c1 = sc * gauss(seed, base | uint64_t(b0 + n + r));
+      if (kind == 0) {
+        if (r == 0) c0 += a.shift;
+        if (r == 1) c1 += a.shift;
+      }
+      v = 0.5 * (c0 + c1) + a.eps * sc * gauss(seed, base | uint64_t(i));
     }
     dst[i] = v;
   }
 }
 
+// one workgroup per system
+template <typename TW>
+__global__ void __launch_bounds__(256) features_batched(const FeatArgs<TW>* args, float* out) {
+  __shared__ double r_min[256], r_max[256], r_sum[256], r_rat[256], r_cnt[256], r_up[256];
+  const FeatArgs<TW>& a = args[blockIdx.x];
+  const int n = a.n, n2 = 2 * n, t = threadIdx.x;
+  double dmin = INFINITY, dmax = 0, lsum = 0, rmin = INFINITY, cnt = 0, umax = 0;
+  for (int j = 1 + t; j < a.N; j += blockDim.x) {   // block j is the pivot block of exactly one pair
+    const TW* T = a.Tm + size_t(j) * tsize(n);
+    double pmin = INFINITY, pmax = 0;
+    for (int c = 0; c < n; ++c) {
+      const double d = fabs(double(T[c + size_t(c) * n2]));
+      pmin = fmin(pmin, d);
+      pmax = fmax(pmax, d);
+      for (int r = 0; r <= c; ++r) umax = fmax(umax, fabs(double(T[r + size_t(c) * n2])));
+    }
+    const double ratio = pmax > 0 ? pmin / pmax : 0.0;
+    dmin = fmin(dmin, pmin);
+    dmax = fmax(dmax, pmax);
+    lsum += log10(ratio + 1e-30);
+    rmin = fmin(rmin, ratio);
+    cnt += ratio < 1e-5 ? 1.0 : 0.0;
+  }
+  r_min[t] = dmin; r_max[t] = dmax; r_sum[t] = lsum; r_rat[t] = rmin; r_cnt[t] = cnt; r_up[t] = umax;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      r_min[t] = fmin(r_min[t], r_min[t + w]);
+      r_max[t] = fmax(r_max[t], r_max[t + w]);
+      r_sum[t] += r_sum[t + w];
+      r_rat[t] = fmin(r_rat[t], r_rat[t + w]);
+      r_cnt[t] += r_cnt[t + w];
+      r_up[t] = fmax(r_up[t], r_up[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    float* f = out + size_t(a.sys) * 12;
+    const int np = a.N - 1;
+    f[0] = float(log2(double(a.N)) / 12.0);
+    f[1] = float(n) / 48.f;
+    f[2] = float(a.qr) / 12.f;
+    f[3] = float(a.nr) / 8.f;
+    if (np > 0) {
+      f[4] = float(log10(r_min[0] + 1e-30));
+      f[5] = float(log10(r_max[0] + 1e-30));
+      f[6] = float(r_sum[0] / np);
+      f[7] = float(log10(r_rat[0] + 1e-30));
+      f[8] = float(r_cnt[0] / np);
+      f[11] = float(log10(r_up[0] / (r_max[0] + 1e-30) + 1e-30));
+    } else {
+      f[4] = f[5] = f[6] = f[7] = f[8] = f[11] = 0.f;
+    }
+    double lmin = INFINITY, lmax = 0;
+    for (int i = 0; i < a.Nc; ++i) {
+      const double d = fabs(a.Hm[i + size_t(i) * a.Nc]);
+      lmin = fmin(lmin, d);
+      lmax = fmax(lmax, d);
+    }
+    f[9] = float(log10((lmax > 0 ? lmin / lmax : 0.0) + 1e-30));
+    const double nb = a.norms[2 * a.sys] > 0 ? a.norms[2 * a.sys] : 1.0;
+    f[10] = float(log10(a.norms[2 * a.sys + 1] / nb + 1e-30));
+  }
+}
+
+__global__ void __launch_bounds__(256) copy_batched(const CopyArgs* args) {
+  const CopyArgs& a = args[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) a.dst[i] = a.src[i];
+}
+
+__global__ void __launch_bounds__(256) add_batched(const CopyArgs* args) {
+  const CopyArgs& a = args[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) a.dst[i] += a.src[i];
+}
+
 }  // namespace
+
+template <typename TW>
+hipError_t launch_features_batched(const FeatArgs<TW>* args, int nsys, float* out, hipStream_t st) {
+  hipLaunchKernelGGL((features_batched<TW>), dim3(nsys), dim3(256), 0, st, args, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_batched(const CopyArgs* args, int nsys, hipStream_t st) {
+  hipLaunchKernelGGL(copy_batched, dim3(nsys), dim3(256), 0, st, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_batched(const CopyArgs* args, int nsys, hipStream_t st) {
+  hipLaunchKernelGGL(add_batched, dim3(nsys), dim3(256), 0, st, args);
+  return hipGetLastError();
+}
+
+template hipError_t launch_features_batched<float>(const FeatArgs<float>*, int, float*, hipStream_t);
+template hipError_t launch_features_batched<double>(const FeatArgs<double>*, int, float*, hipStream_t);
 
 template <typename TA, typename TW>
 hipError_t launch_prep_batched(const PrepArgs<TA, TW>* args, int nsys, hipStream_t st) {

@@ -71,7 +71,19 @@ struct AssembleArgs {
 
 // Last-block factorization methods on the GPU (the select_last choices
 // LU, LUPQ, QR, QRP of BABD_BorderedCR.hxx:118-127, same numbering).
-enum LastMethod : int { kLastLU = 0, kLastLUPQ = 1, kLastQR = 2, kLastQRP = 3 };
+// kLastSVD stands for SVD, LSS, LSY and PINV (4..7): all give the
+// minimum-norm least-squares solution through the SVD.
+enum LastMethod : int { kLastLU = 0, kLastLUPQ = 1, kLastQR = 2, kLastQRP = 3, kLastSVD = 4 };
+
+// Arguments of the SVD last block (see bcr_last_svd.hip).
+struct SvdArgs {
+  int Nr, Nc, conditional;
+  const double* A;          // Nr x Nc, ld Nr
+  double *W, *Vj;           // scratch: max(Nr,Nc) x min(Nr,Nc), min^2
+  double *U, *V, *S;        // Nr x k, Nc x k, k (k = min(Nr, Nc))
+  int32_t* rank;
+  int32_t* status;
+};
 
 // Arguments of the last-block solve: Hm is the factored Nr x Nc block
 // (column-major, ld Nr); P = ipiv (LU, 1-based) / row swaps (LUPQ) / jpvt
@@ -81,6 +93,10 @@ struct LastSolveArgs {
   int N, n, qr, qx, nr, nx, Nr, Nc, method, nrhs;
   const double* Hm; const int32_t* P; const int32_t* Q; const double* tau;
   TX* x; int64_t ldx;
+  // SVD path: taken when method == kLastSVD, or when *status has bit 3 (the
+  // second solver replaced a failed first factorization)
+  const int32_t* status;
+  const double *U, *V, *S; const int32_t* rank;
 };
 
 // Arguments of the FP64 block SpMV (pristine operator).

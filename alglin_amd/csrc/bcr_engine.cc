@@ -168,7 +168,9 @@ void Engine<TA, TW>::level_factor(int L, bool first) {
     using A = std::remove_reference_t<decltype(a)>;
     (void)sizeof(A);
     hipError_t e = hipSuccess;
-    if (method_ == kLU && (launch_pair_lu_split(a, st_, &e) || launch_pair_lu_fast(a, st_, &e))) {
+    const bool lu_fast = method_ == kLU && ((pair_variant_ == 0 && launch_pair_lu_tw(a, st_, &e)) ||
+                                            launch_pair_lu_split(a, st_, &e) || launch_pair_lu_fast(a, st_, &e));
+    if (lu_fast) {
       stats_.fast_path = 1;
       hip_check(e, "pair_lu_fast");
       return;
@@ -197,7 +199,8 @@ void Engine<TA, TW>::factorize_async() {
   if (!pD_) throw HipError("factorize: no operator loaded");
   if ((last_method_ == kLastLU || last_method_ == kLastLUPQ) && d_.Nr() != d_.Nc())
     throw HipError("factorize: the last-block LU needs Nr == Nc (qr+nr == qx+nx)");
-  if (d_.Nr() < d_.Nc()) throw HipError("factorize: the last block needs Nr >= Nc (qr+nr >= qx+nx)");
+  if ((last_method_ == kLastQR || last_method_ == kLastQRP) && d_.Nr() < d_.Nc())
+    throw HipError("factorize: the last-block QR needs Nr >= Nc (qr+nr >= qx+nx)");
   // per-pair LDS tables of the generic and sweep kernels hold up to 64 pivots
   // and 16 border rows
   if (n > 64 || nr > 16) throw HipError("factorize: block size n <= 64 and nr <= 16 are supported");
@@ -241,8 +244,30 @@ void Engine<TA, TW>::factorize_async() {
     hip_check(launch_convert<TA, TW>(pB_, wB_.p, int64_t(bn), st_), "copy B0");
     hip_check(launch_convert<TA, TW>(pC_, wC_.p, int64_t(rn), st_), "copy C0");
   }
+  const bool svd_first = last_method_ >= kLastSVD;
+  if (svd_first || last2_) {
+    const int k = std::min(d_.Nr(), d_.Nc()), rows = std::max(d_.Nr(), d_.Nc());
+    svdW_.resize(size_t(rows) * k);
+    svdVj_.resize(size_t(k) * k);
+    svdU_.resize(size_t(d_.Nr()) * k);
+    svdV_.resize(size_t(d_.Nc()) * k);
+    svdS_.resize(size_t(k));
+    svd_rank_.resize(1);
+    if (!svd_first) Hm2_.resize(size_t(d_.Nr()) * d_.Nc());
+  }
+  SvdArgs sv{};
+  sv.Nr = d_.Nr(); sv.Nc = d_.Nc(); sv.W = svdW_.p; sv.Vj = svdVj_.p; sv.U = svdU_.p; sv.V = svdV_.p;
+  sv.S = svdS_.p; sv.rank = svd_rank_.p; sv.status = status_.p;
   timed(kLast, [&] {
     hip_check(launch_assemble_last(as, st_), "assemble last");
+    if (svd_first) {
+      sv.A = Hm_.p;
+      sv.conditional = 0;
+      hip_check(launch_svd_last(sv, st_), "svd last");
+      return;
+    }
+    if (last2_)
+      hip_check(launch_convert<double, double>(Hm_.p, Hm2_.p, int64_t(d_.Nr()) * d_.Nc(), st_), "copy last block");
     switch (last_method_) {
       case kLastLU: hip_check(launch_lu_last(Hm_.p, d_.Nc(), ipiv_.p, status_.p, lds_limit_, st_), "lu last"); break;
       case kLastLUPQ:
@@ -252,6 +277,11 @@ void Engine<TA, TW>::factorize_async() {
         hip_check(launch_qr_last(last_method_ == kLastQRP, Hm_.p, d_.Nr(), d_.Nc(), tau_.p, ipiv_.p, status_.p,
                                  lds_limit_, st_),
                   "qr last");
+    }
+    if (last2_) {   // the second solver runs on the device only if the first failed
+      sv.A = Hm2_.p;
+      sv.conditional = 1;
+      hip_check(launch_svd_last(sv, st_), "svd last (second solver)");
     }
   });
 }
@@ -307,8 +337,10 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
   timed(kLast, [&] {
     LastSolveArgs<TX> ls{};
     ls.N = N; ls.n = n; ls.qr = d_.qr; ls.qx = d_.qx; ls.nr = nr; ls.nx = d_.nx; ls.Nr = d_.Nr(); ls.Nc = d_.Nc();
-    ls.method = last_method_; ls.nrhs = nrhs; ls.Hm = Hm_.p; ls.P = ipiv_.p; ls.Q = jpiv_.p; ls.tau = tau_.p;
-    ls.x = x; ls.ldx = ldx;
+    ls.method = std::min(last_method_, int(kLastSVD)); ls.nrhs = nrhs; ls.Hm = Hm_.p; ls.P = ipiv_.p;
+    ls.Q = jpiv_.p; ls.tau = tau_.p; ls.x = x; ls.ldx = ldx;
+    ls.status = last2_ ? status_.p : nullptr;
+    ls.U = svdU_.p; ls.V = svdV_.p; ls.S = svdS_.p; ls.rank = svd_rank_.p;
     hip_check(launch_last_solve<TX>(ls, lds_limit_, st_), "last solve");
   });
   int k = Lmax_;

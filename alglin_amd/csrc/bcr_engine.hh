@@ -72,11 +72,17 @@ class Engine {
   void share_stream(hipStream_t s);
 
   void set_method(int m) { method_ = m; }
-  // last-block factorization: kLastLU, kLastLUPQ, kLastQR, kLastQRP
+  // last-block factorization: kLastLU, kLastLUPQ, kLastQR, kLastQRP, or
+  // 4..7 (SVD, LSS, LSY, PINV: the SVD path)
   void set_last_method(int m) { last_method_ = m; }
   int last_method() const { return last_method_; }
+  // second last-block solver (SVD path) used when the first fails
+  void set_last2(bool on) { last2_ = on; }
   // LU solves with n <= 32 use the wave-synchronous sweep kernels unless off
   void set_sweep_fast(bool on) { sweep_fast_ = on; }
+  // LU pair kernel for 16 < n <= 32: 0 two-role waves (panel / columns),
+  // 1 two waves that both factor the panel
+  void set_pair_variant(int v) { pair_variant_ = v; }
   int method() const { return method_; }
 
   // pristine operator: upload from host block arrays (reference layout)
@@ -129,7 +135,9 @@ class Engine {
   Dims d_;
   int method_ = kLU;
   int last_method_ = kLastLU;
+  bool last2_ = false;
   bool sweep_fast_ = true;
+  int pair_variant_ = 0;
   hipStream_t st_ = nullptr;
   bool own_stream_ = true;
   size_t lds_limit_ = 65536;
@@ -143,6 +151,8 @@ class Engine {
   // factors / working
   DevBuf<TW> wD_, wE_, wB_, wC_, Tm_, Cpend_, Fpart_, gscratch_, tmpw_;
   DevBuf<int32_t> perm_, ipiv_, jpiv_, status_;
+  DevBuf<double> Hm2_, svdW_, svdVj_, svdU_, svdV_, svdS_;
+  DevBuf<int32_t> svd_rank_;
   DevBuf<double> Feff_, Hm_, tau_, tmpd_, Cpart_, Csum_, Xpartd_;
   DevBuf<TW> Xpartw_;
   bool pend_clean_ = false;

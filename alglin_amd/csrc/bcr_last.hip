@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(1024) lu_last_kernel(double* __restrict__ Hm, 
 template <typename TX, bool A_LDS>
 __device__ __forceinline__ void last_solve_body(LastSolveArgs<TX> const& a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int n = a.n, Nr = a.Nr, Nc = a.Nc, N = a.N, nrhs = a.nrhs, method = a.method;
+  const int n = a.n, Nr = a.Nr, Nc = a.Nc, N = a.N, nrhs = a.nrhs;
+  const int method = (a.method >= kLastSVD || (a.status && (*a.status & 8))) ? int(kLastSVD) : a.method;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wid = tid >> 6;
   const int ld = Nr;   // column-major Nr x Nc
   const int R = Nr > Nc ? Nr : Nc;
@@ -161,7 +162,27 @@ __device__ __forceinline__ void last_solve_body(LastSolveArgs<TX> const& a) {
       X[r + q * R] = v;
     }
     __syncthreads();
-    if (method == kLastLU || method == kLastLUPQ) {
+    if (method == kLastSVD) {
+      // x = V S^+ U^T b over the rank retained singular triplets
+      const int k = Nr < Nc ? Nr : Nc, rank = *a.rank;
+      for (int i = tid; i < k * nq; i += nth) {
+        const int l = i % k, q = i / k;
+        double c = 0;
+        if (l < rank) {
+          for (int r = 0; r < Nr; ++r) c += a.U[r + size_t(l) * Nr] * X[r + q * R];
+          c /= a.S[l];
+        }
+        Z[l + q * Nc] = c;
+      }
+      __syncthreads();
+      for (int i = tid; i < Nc * nq; i += nth) {
+        const int r = i % Nc, q = i / Nc;
+        double y = 0;
+        for (int l = 0; l < rank; ++l) y += a.V[r + size_t(l) * Nc] * Z[l + q * Nc];
+        X[r + q * R] = y;
+      }
+      __syncthreads();
+    } else if (method == kLastLU || method == kLastLUPQ) {
       for (int q = tid; q < nq; q += nth)
         for (int c = 0; c < Nc; ++c) {
           const int pv = method == kLastLU ? a.P[c] - 1 : a.P[c];
@@ -200,7 +221,7 @@ __device__ __forceinline__ void last_solve_body(LastSolveArgs<TX> const& a) {
       }
     }
     // R^{-1} (or U^{-1}), bottom up
-    for (int c = Nc - 1; c >= 0; --c) {
+    for (int c = method == kLastSVD ? -1 : Nc - 1; c >= 0; --c) {
       const double uinv = 1.0 / A[c + c * ld];
       for (int i = tid; i < c * nq; i += nth) {
         const int r = i % c, q = i / c;

@@ -16,6 +16,8 @@ template <typename TS, typename TW>
 bool launch_pair_lu_fast(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 bool launch_pair_lu_split(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
+bool launch_pair_lu_tw(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 
 template <typename TW, typename TX>
 hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
@@ -35,6 +37,7 @@ template <typename TX>
 hipError_t launch_last_solve(LastSolveArgs<TX> const& a, size_t lds_limit, hipStream_t st);
 hipError_t launch_lupq_last(double* Hm, int Nc, int32_t* P, int32_t* Q, int32_t* status, size_t lds_limit,
                             hipStream_t st);
+hipError_t launch_svd_last(SvdArgs const& a, hipStream_t st);
 hipError_t launch_qr_last(bool pivot, double* Hm, int Nr, int Nc, double* tau, int32_t* jpvt, int32_t* status,
                           size_t lds_limit, hipStream_t st);
 

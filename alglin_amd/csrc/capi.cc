@@ -11,6 +11,7 @@
 
 #include "BorderedCR.hh"
 #include "bcr_batch.hh"
+#include "bcr_mlp.hh"
 
 using alglin_amd::BorderedCR;
 
@@ -293,6 +294,17 @@ int alglin_bcr_solve_device(alglin_bcr_t h, void* X, int32_t nrhs, int64_t ld, i
 int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32_t max_iter) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) { s.select_precision(mode ? mode : "FP64", tol, max_iter); return 0; });
+  });
+}
+
+int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto*) {
+      s.engine().set_pair_variant(variant);
+      if constexpr (std::is_same_v<std::remove_reference_t<decltype(s)>, alglin_amd::BorderedCR<double>>)
+        if (s.mixed_engine()) s.mixed_engine()->engine().set_pair_variant(variant);
+      return 0;
+    });
   });
 }
 
@@ -619,6 +631,14 @@ int alglin_batch_generate(alglin_batch_t h, uint64_t seed, double shift) {
   });
 }
 
+int alglin_batch_generate_singular(alglin_batch_t h, uint64_t seed, double shift, int32_t nsys,
+                                   int32_t const* sing_k, double eps) {
+  return bguard(h, [&] {
+    h->s->generate(seed, shift, std::vector<int32_t>(sing_k, sing_k + nsys), eps);
+    return 0;
+  });
+}
+
 int alglin_batch_solve(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter, int use_graph) {
   return bguard(h, [&] {
     h->rep = h->s->solve(B, X, tol, max_iter, use_graph != 0);
@@ -649,5 +669,128 @@ int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual,
 char const* alglin_batch_last_error(alglin_batch_t h) { return h ? h->err.c_str() : "null handle"; }
 
 void* alglin_batch_stream(alglin_batch_t h) { return h ? static_cast<void*>(h->s->stream()) : nullptr; }
+
+int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods) {
+  return bguard(h, [&] {
+    h->s->set_methods(std::vector<int32_t>(methods, methods + nsys));
+    return 0;
+  });
+}
+
+// ---- learned selector ----------------------------------------------------------
+struct alglin_mlp {
+  std::unique_ptr<bcr::Mlp> m;
+  std::string err{"no error"};
+};
+
+int32_t alglin_mlp_num_params(void) { return bcr::kMlpParams; }
+
+alglin_mlp_t alglin_mlp_create(uint64_t seed) {
+  try {
+    auto* h = new alglin_mlp;
+    h->m.reset(new bcr::Mlp(seed));
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void alglin_mlp_destroy(alglin_mlp_t h) { delete h; }
+
+char const* alglin_mlp_last_error(alglin_mlp_t h) { return h ? h->err.c_str() : "null handle"; }
+
+}  // extern "C"
+
+namespace {
+template <typename Fn>
+int mguard(alglin_mlp_t h, Fn&& fn) {
+  if (!h) return 1;
+  try {
+    return fn();
+  } catch (std::exception const& e) {
+    h->err = e.what();
+    return 1;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int alglin_mlp_get_params(alglin_mlp_t h, float* params, float* mu, float* sig) {
+  return mguard(h, [&] {
+    if (params) h->m->get_params(params);
+    if (mu && sig) h->m->get_normalization(mu, sig);
+    return 0;
+  });
+}
+
+int alglin_mlp_set_params(alglin_mlp_t h, float const* params, float const* mu, float const* sig) {
+  return mguard(h, [&] {
+    if (params) h->m->set_params(params);
+    if (mu && sig) h->m->set_normalization(mu, sig);
+    return 0;
+  });
+}
+
+int alglin_mlp_load(alglin_mlp_t h, char const* path) {
+  return mguard(h, [&] {
+    h->m->load(path);
+    return 0;
+  });
+}
+
+int alglin_mlp_save(alglin_mlp_t h, char const* path) {
+  return mguard(h, [&] {
+    h->m->save(path);
+    return 0;
+  });
+}
+
+int alglin_mlp_loss_grad(alglin_mlp_t h, float const* X, int32_t const* y, int32_t n, float* loss, float* grad) {
+  return mguard(h, [&] {
+    const float l = h->m->loss_grad(X, y, n, grad);
+    if (loss) *loss = l;
+    return 0;
+  });
+}
+
+int alglin_mlp_train(alglin_mlp_t h, float const* X, int32_t const* y, int32_t n, int32_t epochs, float lr,
+                     float* losses) {
+  return mguard(h, [&] {
+    auto hist = h->m->train(X, y, n, epochs, lr);
+    if (losses) std::memcpy(losses, hist.data(), hist.size() * sizeof(float));
+    return 0;
+  });
+}
+
+int alglin_mlp_predict(alglin_mlp_t h, float const* X, int32_t n, float* p1) {
+  return mguard(h, [&] {
+    h->m->predict(X, n, p1);
+    return 0;
+  });
+}
+
+int alglin_batch_solve_selected(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter,
+                                alglin_mlp_t selector, float threshold, int fallback) {
+  return bguard(h, [&] {
+    h->rep = h->s->solve_selected(B, X, tol, max_iter, selector ? selector->m.get() : nullptr, threshold,
+                                  fallback != 0);
+    return h->rep.failed == 0 ? 0 : 2;
+  });
+}
+
+int alglin_batch_selection(alglin_batch_t h, float* features, int32_t* chose_qrp, int32_t* fell_back) {
+  return bguard(h, [&] {
+    auto const& f = h->s->features();
+    auto const& q = h->s->chose_qrp();
+    auto const& b = h->s->fell_back();
+    if (features) std::memcpy(features, f.data(), f.size() * sizeof(float));
+    for (size_t i = 0; i < q.size(); ++i) {
+      if (chose_qrp) chose_qrp[i] = q[i];
+      if (fell_back) fell_back[i] = b[i];
+    }
+    return 0;
+  });
+}
 
 }  // extern "C"

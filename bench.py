@@ -15,7 +15,12 @@ baseline: the oracle's threaded restatement of the reference algorithm
 (range partition over all host threads) timed on a bounded sample of the
 same workload on this machine's host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Other workloads (--workload batched | selector): thousands of uneven systems
+in one batched solve (FP32 factors + FP64 refinement, graph replay), and the
+learned LU/QRP selector on a mix of regular and nearly singular systems;
+both against plain FP64 on the same batch.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload single]
 """
 from __future__ import annotations
 
@@ -89,6 +94,178 @@ def run_cpu_baseline(N, n, qr, qx, nr, nx, shift, seed, sample_N, reps=3):
                 seconds_sample=t)
 
 
+def batch_pair_bytes(dims, s):
+    """Algorithmic bytes of every pair elimination of a batch (factor element size s)."""
+    tot = 0.0
+    for N, n, qr, qx, nr, nx, _ in dims:
+        tot += float(N - 1) * pair_bytes(int(n), int(nr), int(nx), s)
+    return tot
+
+
+def batch_sweep_bytes(dims, s):
+    """Bytes of one forward + backward sweep (factors read once, x twice) and
+    one FP64 residual (operator read once) of a batch."""
+    f = r = 0.0
+    for N, n, qr, qx, nr, nx, _ in dims:
+        N, n, nr, nx, qr, qx = int(N), int(n), int(nr), int(nx), int(qr), int(qx)
+        f += (N - 1) * (s * (2 * n * n + n) + s * (3 * n * n + n * nx) + 4 * n) + 8 * 4 * n * (N + 1)
+        nnz = n * N * (2 * n + nx + nr) + nr * (n + qx + nx) + (n + qr) * (2 * n + qx + nx)
+        r += 8.0 * nnz + 16.0 * (n * (N + 1) + qr + nr)
+    return f, r
+
+
+def run_cpu_baseline_batch(bt, dims, sample, reps=2):
+    """The oracle on `sample` systems of the batch, one system per host thread
+    (num_parallel_block = 1 each, as the reference runs independent systems),
+    scaled to the whole batch by the N n^3 work ratio."""
+    from concurrent.futures import ThreadPoolExecutor
+    from alglin_amd import systems
+    from oracle import pyoracle
+    pyoracle.build()
+    P = cpu_threads()
+    sysd = [bt.system(s) for s in range(sample)]
+    rhs = [np.random.default_rng(s).standard_normal(systems.dims(d)[0]) for s, d in enumerate(sysd)]
+
+    def one(i):
+        d = sysd[i]
+        o = pyoracle.OracleBorderedCR(np.float64, 1, 1)
+        o.allocate(d["N"], d["n"], d["qr"], d["qx"], d["nr"], d["nx"])
+        o.load(d)
+        o.select("LU", "LU")
+        ok = o.factorize()
+        ok2, _ = o.solve(rhs[i])
+        return ok and ok2
+
+    times = []
+    with ThreadPoolExecutor(P) as ex:
+        for rep in range(reps + 1):
+            t0 = time.perf_counter()
+            assert all(ex.map(one, range(sample)))
+            if rep > 0:
+                times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    w = (dims[:, 0].astype(float) * dims[:, 1].astype(float) ** 3)
+    scale = w.sum() / w[:sample].sum()
+    return dict(value=len(dims) / (t * scale), unit="solves/s", cores=P, kind="port",
+                sample=f"oracle C++ restatement, {sample} systems of the batch on {P} threads (one system per "
+                       f"thread, FP64 LU), median of {reps} runs = {t:.3f} s, scaled x{scale:.1f} by N*n^3 work",
+                seconds_sample=t)
+
+
+def bench_batched(args):
+    """Batched uneven systems: plain FP64 cyclic reduction vs FP32 factors +
+    FP64 refinement (graph replay), or, for the selector workload, vs the
+    mixed solve with the learned LU/QRP choice and the FP64 QRP fallback."""
+    from alglin_amd import selector as S
+    from alglin_amd import systems
+    from alglin_amd.bcr import Batch, DeviceArray, device_info
+    sel = args.workload == "selector"
+    nsys = args.nsys or (8192 if sel else 4096)
+    t_gen = time.perf_counter()
+    if sel:
+        dims = S.selector_dims(nsys, args.seed)
+        sing = S.singular_pairs(dims, args.seed)
+        eps = 1e-10
+    else:
+        dims = systems.uneven_batch_dims(nsys, args.seed)
+        sing, eps = None, 0.0
+    ref = Batch(mixed=False)
+    ref.plan(dims)
+    ref.generate(seed=args.seed, shift=4.0, sing_k=sing, eps=eps)
+    mix = Batch(mixed=True)
+    mix.plan(dims)
+    mix.share_operator(ref)
+    L = ref.size("x")
+    B = DeviceArray.from_host(np.random.default_rng(args.seed + 5).standard_normal(L))
+    X = DeviceArray(L)
+    print(f"[bench] {nsys} systems generated on the GPU in {time.perf_counter() - t_gen:.1f} s "
+          f"(operator {8 * (2 * ref.size('DE') + ref.size('B') + ref.size('C')) / 1e9:.1f} GB)",
+          file=sys.stderr, flush=True)
+
+    def timed(fn, steps, warm):
+        for _ in range(warm):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t0) / steps
+
+    tol = 1e-12
+    t64 = timed(lambda: ref.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
+    r64 = ref.report()
+    out = {}
+    if sel:
+        mlp = S.load()
+        t_none = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=None, fallback=True), args.steps,
+                       args.warmup)
+        r_none = mix.report()
+        fb_none = int(mix.selection()["fell_back"].sum())
+        tmix = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=True), args.steps, args.warmup)
+        rmix = mix.report()
+        chosen = mix.selection()
+        out["mixed_without_selector"] = dict(solves_per_s=nsys / t_none, ms=1e3 * t_none, fp64_fallbacks=fb_none,
+                                             max_sweeps=r_none["max_sweeps"],
+                                             max_rel_residual=r_none["max_rel_residual"])
+        out["selector"] = dict(sent_to_qrp=int(chosen["chose_qrp"].sum()),
+                               fp64_fallbacks=int(chosen["fell_back"].sum()),
+                               singular_systems=int((sing >= 1).sum()))
+    else:
+        tmix = timed(lambda: mix.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
+        rmix = mix.report()
+    sweeps = int(rmix["max_sweeps"])
+    fac = batch_pair_bytes(dims, 4)
+    sw, res = batch_sweep_bytes(dims, 4)
+    nbytes = fac + (1 + sweeps) * sw + (1 + sweeps) * res
+    achieved = nbytes / tmix / 1e9
+    print(f"[bench] plain FP64 {1e3 * t64:.1f} ms, mixed{'+learned' if sel else ''} {1e3 * tmix:.1f} ms, "
+          f"max residual {rmix['max_rel_residual']:.2e}", file=sys.stderr, flush=True)
+    result = {
+        "metric": "batched BABD solves/sec @ <=1e-12 rel. residual, mixed+learned vs plain FP64",
+        "value": nsys / tmix,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * tmix,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factors + f64 refinement",
+        "data": "synthetic (counter-based Gaussian blocks generated on the GPU, random b)",
+        "config": {
+            "workload": (f"{nsys} uneven BABD systems, n in {{8,16,32}}, N in [64,2048], half with a rank-deficient "
+                         f"pair column (1e-10), mixed + learned LU/QRP selector + FP64 QRP fallback" if sel else
+                         f"{nsys} uneven BABD systems, n in 4..48, N log-uniform [16,4096], qr=qx=n/4, nr=nx=2, "
+                         f"shift 4, FP32 factors + FP64 refinement, graph replay"),
+            "nsys": nsys, "tol": tol, "parallelism": "single-gpu",
+        },
+        "plain_fp64": dict(solves_per_s=nsys / t64, ms=1e3 * t64, max_rel_residual=r64["max_rel_residual"],
+                           failed=r64["failed"]),
+        "speedup_vs_plain_fp64": t64 / tmix,
+        "rel_residual_max": rmix["max_rel_residual"],
+        "failed_systems": rmix["failed"],
+        "refinement_sweeps_max": sweeps,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "whole batched solve (pair eliminations + sweeps + FP64 residuals), algorithmic bytes / wall time",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_step": nbytes,
+        },
+        "device": device_info(),
+    }
+    result.update(out)
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = run_cpu_baseline_batch(ref, dims, min(nsys, args.cpu_sample_sys))
+        except Exception as e:  # the baseline is reported, never required
+            result["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,12 +283,20 @@ def main():
                     help="fp64: plain FP64 cyclic reduction; mixed: FP32 factors + FP64 refinement to 1e-12")
     ap.add_argument("--kappa", type=float, default=0.0,
                     help="if > 0, D_k = Q diag(s) Q^T with cond(D_k) log-uniform in [1e2, kappa]")
+    ap.add_argument("--workload", choices=["single", "batched", "selector"], default="single",
+                    help="single: one large system; batched: 4096 uneven systems; selector: 8192 systems with "
+                         "the learned LU/QRP choice")
+    ap.add_argument("--nsys", type=int, default=0)
+    ap.add_argument("--cpu-sample-sys", type=int, default=48)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (each box has one MI355X); "
               "--gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "single":
+        bench_batched(args)
+        return
 
     from alglin_amd import systems
     from alglin_amd.bcr import BorderedCR, DeviceArray, device_info, device_synchronize, packed_blocks

@@ -112,6 +112,9 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T, int32_t* perm, void* D, void
                            double* last_block, double* pending_sumsq);
 /* 1 when the last factorization used the register-resident LU kernel */
 int alglin_bcr_fast_path(alglin_bcr_t h);
+/* LU pair kernel for 16 < n <= 32: 0 (default) panel wave + columns wave,
+   1 two waves that both factor the panel (for comparison) */
+int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant);
 
 /* ---- precision (extension of the reference surface) -------------------------- */
 /* mode "FP64" (default, the reference's arithmetic) or "MIXED" (double handles
@@ -174,6 +177,11 @@ int alglin_batch_bind_device(alglin_batch_t h, double const* D, double const* E,
 /* fill the bound arrays with seeded Gaussian systems: D_k = shift I + G/sqrt(n),
    E_k = G/sqrt(n), the other blocks G (counter-based generator on the GPU) */
 int alglin_batch_generate(alglin_batch_t h, uint64_t seed, double shift);
+/* as above; for system s with 1 <= sing_k[s] < N the pair column block
+   [E_{k-1}; D_k] (k = sing_k[s]) is made rank deficient up to eps: its last
+   column is the mean of its first two plus eps G/sqrt(n) (n >= 3) */
+int alglin_batch_generate_singular(alglin_batch_t h, uint64_t seed, double shift, int32_t nsys,
+                                   int32_t const* sing_k, double eps);
 /* X = A^{-1} B for every system (device pointers, kind 6 layout); returns 0
    when every system reached ||b - A x||/||b|| <= tol, 2 when some did not
    (see alglin_batch_report), 1 on error */
@@ -184,6 +192,44 @@ int alglin_batch_solve(alglin_batch_t h, double const* B, double* X, double tol,
 int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual, int32_t* sweeps, double summary[4]);
 char const* alglin_batch_last_error(alglin_batch_t h);
 void* alglin_batch_stream(alglin_batch_t h);
+
+/* per-system pair elimination method for alglin_batch_solve (0 LU, 1 QR, 2 QRP) */
+int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods);
+
+/* ---- learned LU-vs-QRP selector (extension: no counterpart in the reference)
+   A 12 -> 64 -> 64 -> 2 ReLU perceptron, FP32, trained and evaluated on the
+   GPU.  Features (12 per system) come from the FP32 LU factors and the first
+   residual (alglin_batch_selection); class 1 = eliminate with QRP.  Parameter
+   vector of alglin_mlp_num_params() floats: W1 (12x64 row-major), b1, W2
+   (64x64), b2, W3 (64x2), b3; inputs standardized by (x - mu) / sig. */
+typedef struct alglin_mlp* alglin_mlp_t;
+int32_t alglin_mlp_num_params(void);
+alglin_mlp_t alglin_mlp_create(uint64_t seed);
+void alglin_mlp_destroy(alglin_mlp_t h);
+char const* alglin_mlp_last_error(alglin_mlp_t h);
+int alglin_mlp_get_params(alglin_mlp_t h, float* params, float* mu, float* sig);
+int alglin_mlp_set_params(alglin_mlp_t h, float const* params, float const* mu, float const* sig);
+/* weights file: 12 mu, 12 sig, then the parameters (float32) */
+int alglin_mlp_load(alglin_mlp_t h, char const* path);
+int alglin_mlp_save(alglin_mlp_t h, char const* path);
+/* mean cross-entropy of n samples (X: n x 12 device floats, y: n device
+   int32 labels) and its gradient (host, alglin_mlp_num_params floats) */
+int alglin_mlp_loss_grad(alglin_mlp_t h, float const* X, int32_t const* y, int32_t n, float* loss, float* grad);
+/* full-batch Adam for `epochs` steps; losses (host, epochs floats) may be null */
+int alglin_mlp_train(alglin_mlp_t h, float const* X, int32_t const* y, int32_t n, int32_t epochs, float lr,
+                     float* losses);
+/* probability of class 1 per sample (device output) */
+int alglin_mlp_predict(alglin_mlp_t h, float const* X, int32_t n, float* p1);
+
+/* mixed batch solve with per-system method choice: FP32 LU first; systems the
+   selector scores above threshold (selector may be null: none) are
+   refactored with FP32 QRP; FP64 refinement; with fallback=1 the systems that
+   still miss tol are solved by FP64 QRP.  Return codes as alglin_batch_solve. */
+int alglin_batch_solve_selected(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter,
+                                alglin_mlp_t selector, float threshold, int fallback);
+/* of the last solve_selected: features (12 per system), systems sent to QRP,
+   systems solved by the FP64 fallback (host arrays; any may be null) */
+int alglin_batch_selection(alglin_batch_t h, float* features, int32_t* chose_qrp, int32_t* fell_back);
 
 #ifdef __cplusplus
 }

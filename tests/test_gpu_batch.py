@@ -69,7 +69,8 @@ def test_batch_fp64_matches_oracle_per_system(oracle_mod):
         err = np.abs(xs - xo[:ncols]).max() / max(1.0, np.abs(xo[:ncols]).max())
         assert err < 1e-10, (s, tuple(row), err)
         res = np.linalg.norm(systems.mv(sysd, xs) - bs) / np.linalg.norm(bs)
-        assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 1e-3 * res + 1e-17, (s, res)
+        # the GPU's own FP64 residual agrees with numpy's to rounding
+        assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 1e-3 * res + 1e-15, (s, res)
 
 
 def test_batch_mixed_reaches_tolerance_and_graph_is_exact():

@@ -232,3 +232,27 @@ def test_device_resident_path():
     g.solve_device(X, 1, nrows)
     t = g.timing()
     assert t["pair_launches"] == 9 and t["pair_ms"] > 0 and t["forward_ms"] > 0
+
+
+@pytest.mark.parametrize("shape", [(19, 32, 8, 8, 4, 4), (40, 24, 3, 3, 8, 8), (33, 17, 2, 2, 1, 1)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_pair_kernel_variants_bit_identical(shape, dtype):
+    """The two LU pair kernels for 16 < n <= 32 (panel wave + columns wave,
+    and two waves that both factor the panel) do the same arithmetic in the
+    same order: identical factors and solutions."""
+    N, n, qr, qx, nr, nx = shape
+    s = systems.gaussian_system(N, n, qr, qx, nr, nx, shift=3.0, seed=N + n)
+    nrows, _ = systems.dims(s)
+    b = np.random.default_rng(0).standard_normal(nrows).astype(dtype)
+    out = []
+    for variant in (0, 1):
+        g = gpu_solver(s, "LU", dtype)
+        g.set_pair_kernel(variant)
+        assert g.factorize(), g.last_error()
+        ok, x = g.solve(b)
+        assert ok
+        out.append((x, g.factors()))
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in out[0][1]:
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+
