@@ -270,7 +270,7 @@ class BorderedCR:
         self._check(lib().alglin_bcr_set_precision(self._h, mode.encode(), tol, max_iter), "select_precision")
 
     def set_pair_kernel(self, variant: int):
-        """LU pair kernel for 16 < n <= 32: 0 panel wave + columns wave, 1 both waves factor the panel."""
+        """LU pair kernel: 0 default, 2 two-role kernel (16 < n <= 32), 3 register-tiled for every n."""
         self._check(lib().alglin_bcr_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def set_fallback(self, on: bool):

@@ -1,6 +1,8 @@
 // Host side of alglin_amd::BorderedCR<T> (see BorderedCR.hh).
 #include "BorderedCR.hh"
 
+#include "bcr_launch.hh"
+
 #include <cmath>
 #include <sstream>
 #include <stdexcept>
@@ -384,7 +386,26 @@ void BorderedCR<T>::fallback_qrp(T* X, integer nrhs, int64_t ld) {
                                     size_t(L) * sizeof(double), nrhs, hipMemcpyDeviceToDevice, eng_->stream()),
                    "fallback rhs");
     eng_->template solve_async<double>(X, nrhs, ld);
-    report_.rel_residual = mix_->max_rel_residual(X, nrhs, ld, mix_->rhs(), L);
+    // FP64 refinement on the FP64 QRP factors (they lose digits on nearly
+    // singular pair columns; the sweeps recover them)
+    double rel = mix_->max_rel_residual(X, nrhs, ld, mix_->rhs(), L);
+    const int64_t nrows = d_.nrows();
+    for (int it = 0; it < ir_max_ && rel > ir_tol_ && rel == rel; ++it) {
+      double* R = mix_->residuals();
+      if (L > nrows)
+        for (integer c = 0; c < nrhs; ++c)
+          bcr::hip_check(bcr::launch_convert<double, double>(nullptr, R + size_t(c) * L + nrows, L - nrows,
+                                                              eng_->stream()),
+                         "pad");
+      eng_->template solve_async<double>(R, nrhs, L);
+      for (integer c = 0; c < nrhs; ++c)
+        bcr::hip_check(bcr::launch_axpy(1.0, R + size_t(c) * L, X + size_t(c) * ld, d_.ncols(), eng_->stream()),
+                       "x += d");
+      const double prev = rel;
+      rel = mix_->max_rel_residual(X, nrhs, ld, mix_->rhs(), L);
+      if (!(rel < 0.9 * prev)) break;
+    }
+    report_.rel_residual = rel;
     report_.status = 4;
   }
 }

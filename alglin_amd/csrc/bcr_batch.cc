@@ -349,7 +349,9 @@ void BatchEngine<TA, TW>::factorize_async() {
         BatchTable<A> t{args, l.entry, l.first};
         bool ok = true;
         if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
-        else if (l.cls == 3) ok = launch_pair_lu_tw_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
+        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
+        else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
+          ok = launch_pair_lu_tiled_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
         else {
           const size_t bytes = size_t(2 * l.nmax + l.nrmax) * (3 * l.nmax + l.nxmax) * sizeof(TW);
           e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, st_);
@@ -401,7 +403,7 @@ typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(dou
         args.push_back(a);
         nmax = std::max(nmax, d.n);
         nxmax = std::max(nxmax, d.nx);
-        const int ppb = cls <= 3 ? sweep_lu_fast_pairs_per_block(kSweepNmax[cls]) : 1;
+        const int ppb = cls <= 3 ? sweep_lu_fast_pairs_per_block(kSweepNmax[cls]) : (cls == 4 && d.n <= 64 ? 4 : 1);
         const int blocks = (a.npairs + ppb - 1) / ppb;
         const int e = int(first.size()) - 1;
         for (int b = 0; b < blocks; ++b) entry.push_back(e);
@@ -462,6 +464,9 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
     hipError_t e = hipSuccess;
     if (l.cls <= 3) {
       if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, st_, &e))
+        throw HipError("batch solve: no sweep kernel");
+    } else if (l.cls == 4) {
+      if (!launch_sweep_lu_stream_batched(fwd, t, l.nblocks, l.nmax, st_, &e))
         throw HipError("batch solve: no sweep kernel");
     } else {
       e = launch_sweep_generic_batched(fwd, l.method, t, l.nblocks, l.nmax, l.nxmax, st_);

@@ -85,6 +85,9 @@ template <typename TS, typename TW>
 bool launch_pair_lu_split_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                   hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
+bool launch_pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
+                                  hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
 bool launch_pair_lu_tw_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
@@ -96,6 +99,9 @@ int sweep_lu_fast_pairs_per_block(int n);
 template <typename TW, typename TX>
 bool launch_sweep_lu_fast_batched(bool forward, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks, int n, int nrhs,
                                   hipStream_t st, hipError_t* err);
+template <typename TW, typename TX>
+bool launch_sweep_lu_stream_batched(bool forward, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks, int n,
+                                    hipStream_t st, hipError_t* err);
 template <typename TW, typename TX>
 hipError_t launch_sweep_generic_batched(bool forward, int method, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks,
                                         int n, int nx, hipStream_t st);

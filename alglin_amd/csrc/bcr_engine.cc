@@ -168,8 +168,10 @@ void Engine<TA, TW>::level_factor(int L, bool first) {
     using A = std::remove_reference_t<decltype(a)>;
     (void)sizeof(A);
     hipError_t e = hipSuccess;
-    const bool lu_fast = method_ == kLU && ((pair_variant_ == 0 && launch_pair_lu_tw(a, st_, &e)) ||
-                                            launch_pair_lu_split(a, st_, &e) || launch_pair_lu_fast(a, st_, &e));
+    const bool lu_fast = method_ == kLU && ((pair_variant_ == 2 && launch_pair_lu_tw(a, st_, &e)) ||
+                                            (pair_variant_ != 3 && launch_pair_lu_split(a, st_, &e)) ||
+                                            (pair_variant_ != 3 && launch_pair_lu_fast(a, st_, &e)) ||
+                                            launch_pair_lu_tiled(a, st_, &e));
     if (lu_fast) {
       stats_.fast_path = 1;
       hip_check(e, "pair_lu_fast");
@@ -323,7 +325,9 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
     a.npairs = (N + L - 1) / (2 * L);
     timed(kForward, [&] {
       hipError_t e = hipSuccess;
-      if (method_ == kLU && sweep_fast_ && launch_sweep_lu_fast(true, a, st_, &e)) hip_check(e, "forward_fast");
+      if (method_ == kLU && sweep_fast_ &&
+          (launch_sweep_lu_fast(true, a, st_, &e) || launch_sweep_lu_stream(true, a, st_, &e)))
+        hip_check(e, "forward_fast");
       else hip_check(launch_forward(method_, a, st_), "forward");
     });
     L *= 2;
@@ -349,7 +353,9 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
     a.npairs = (N + k - 1) / (2 * k);
     timed(kBackward, [&] {
       hipError_t e = hipSuccess;
-      if (method_ == kLU && sweep_fast_ && launch_sweep_lu_fast(false, a, st_, &e)) hip_check(e, "backward_fast");
+      if (method_ == kLU && sweep_fast_ &&
+          (launch_sweep_lu_fast(false, a, st_, &e) || launch_sweep_lu_stream(false, a, st_, &e)))
+        hip_check(e, "backward_fast");
       else hip_check(launch_backward(method_, a, st_), "backward");
     });
   }

@@ -80,8 +80,9 @@ class Engine {
   void set_last2(bool on) { last2_ = on; }
   // LU solves with n <= 32 use the wave-synchronous sweep kernels unless off
   void set_sweep_fast(bool on) { sweep_fast_ = on; }
-  // LU pair kernel for 16 < n <= 32: 0 two-role waves (panel / columns),
-  // 1 two waves that both factor the panel
+  // LU pair kernel choice: 0 default (register kernels for n <= 32, the
+  // register-tiled kernel above), 2 the two-role kernel for 16 < n <= 32,
+  // 3 the register-tiled kernel for every n
   void set_pair_variant(int v) { pair_variant_ = v; }
   int method() const { return method_; }
 

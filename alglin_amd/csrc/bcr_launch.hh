@@ -18,11 +18,18 @@ template <typename TS, typename TW>
 bool launch_pair_lu_split(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 bool launch_pair_lu_tw(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
+// register-tiled LU for any n <= 64 (2n + nr <= 160, 3n + nx <= 208)
+bool pair_lu_tiled_fits(int n, int nr, int nx);
+template <typename TS, typename TW>
+bool launch_pair_lu_tiled(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 
 template <typename TW, typename TX>
 hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
 template <typename TW, typename TX>
 hipError_t launch_backward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);
+// streamed LU sweeps, one wavefront per pair (n <= 64)
+template <typename TW, typename TX>
+bool launch_sweep_lu_stream(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st, hipError_t* err);
 // wave-synchronous LU sweeps (n <= 32); false when the shape is not covered
 template <typename TW, typename TX>
 bool launch_sweep_lu_fast(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st, hipError_t* err);

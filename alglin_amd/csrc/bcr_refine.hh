@@ -37,8 +37,10 @@ class MixedEngine {
   // the right-hand sides of the last solve (ld = max(nrows, ncols))
   double const* rhs() const { return b_.p; }
   int64_t rhs_ld() const;
-  // max over columns of ||B - A X|| / ||B|| (FP64, synchronizes the stream)
+  // max over columns of ||B - A X|| / ||B|| (FP64, synchronizes the stream);
+  // the residual columns stay in residuals() (ld rhs_ld())
   double max_rel_residual(double const* X, int nrhs, int64_t ld, double const* B, int64_t ldb);
+  double* residuals() { return r_.p; }
 
  private:
   void norms_async(double const* X, int nrhs, int64_t ld, double const* B, int64_t ldb);

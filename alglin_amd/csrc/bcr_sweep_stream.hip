@@ -1,0 +1,177 @@
+// Solve sweeps for LU-eliminated pairs with n <= 64, one wavefront per pair
+// (four per workgroup), the factor blocks streamed from HBM column by column
+// instead of held in registers or staged in LDS.
+//
+//   forward  (cc:1615-1719): lane l owns rows l and l + 64 of W = [x_jp; x_j]
+//            (2n <= 128 rows).  The row interchanges are composed into a
+//            source row per lane and applied while gathering W from x; then
+//            W <- [L 0; M I]^{-1} W with x_c broadcast by v_readlane (row c < n
+//            <= 64 sits in lane c); x_j <- W[0:n], x_jp <- W[n:2n];
+//            Xpart[j] = -C_j' x_j (wave sums).
+//   backward (cc:1805-1949): lane r < n owns row r:
+//            s = x_j - D_j x_jp - E_j x_jpp - B_j xb (the vectors broadcast from
+//            the lanes that hold them), then x_j = U^{-1} s bottom up.
+#include "bcr_launch.hh"
+#include "bcr_batch_kernels.hh"
+
+namespace bcr {
+namespace {
+
+__device__ __forceinline__ double rlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <typename TW, typename TX>
+__device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int bid) {
+  const int lane = threadIdx.x & 63;
+  const int p = bid * 4 + int(threadIdx.x >> 6);
+  if (p >= a.npairs) return;   // uniform over the wavefront
+  const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
+  int j, jp, jpp;
+  pair_index(p, a.L, a.N, j, jp, jpp);
+  const TW* Tj = a.Tm + size_t(j) * tsize(n);
+  const int r0 = lane, r1 = lane + 64;
+  const bool v0 = r0 < n2, v1 = r1 < n2;
+  // source rows after the interchanges (laswp over rows 1..n, 1-based ipiv)
+  int src0 = r0, src1 = r1;
+  const int pvl = lane < n ? a.perm[size_t(j) * n + lane] - 1 : 0;
+  for (int c = 0; c < n; ++c) {
+    const int pv = __builtin_amdgcn_readlane(pvl, c);
+    if (pv == c) continue;
+    const int sc = __builtin_amdgcn_readlane(src0, c);
+    const int sp = pv < 64 ? __builtin_amdgcn_readlane(src0, pv) : __builtin_amdgcn_readlane(src1, pv - 64);
+    if (lane == c) src0 = sp;
+    if (pv < 64) { if (lane == pv) src0 = sc; }
+    else if (lane == pv - 64) src1 = sc;
+  }
+  auto in_row = [&](int r) { return r < n ? size_t(jp) * n + r : size_t(j) * n + (r - n); };
+  auto out_row = [&](int r) { return r < n ? size_t(j) * n + r : size_t(jp) * n + (r - n); };
+  const TW* Cj = a.C + size_t(j) * nr * n;
+  for (int q = 0; q < nrhs; ++q) {
+    TX* x = a.x + size_t(q) * a.ldx;
+    TX x0 = v0 ? x[in_row(src0)] : TX(0);
+    TX x1 = v1 ? x[in_row(src1)] : TX(0);
+    TW t0 = v0 ? Tj[r0] : TW(0), t1 = v1 ? Tj[r1] : TW(0);
+    for (int c = 0; c < n; ++c) {
+      // next column's factors in flight while this one is applied
+      TW n0 = TW(0), n1 = TW(0);
+      if (c + 1 < n) {
+        n0 = v0 ? Tj[r0 + size_t(c + 1) * n2] : TW(0);
+        n1 = v1 ? Tj[r1 + size_t(c + 1) * n2] : TW(0);
+      }
+      const TX xc = TX(rlane(double(x0), c));
+      if (r0 > c) x0 -= TX(t0) * xc;
+      x1 -= TX(t1) * xc;
+      t0 = n0;
+      t1 = n1;
+    }
+    if (v0) x[out_row(r0)] = x0;
+    if (v1) x[out_row(r1)] = x1;
+    for (int i = 0; i < nr; ++i) {
+      const double s = wsum(lane < n ? double(Cj[i + size_t(lane) * nr]) * double(x0) : 0.0);
+      if (lane == 0) a.Xpart[size_t(j) * nr * nrhs + i + size_t(q) * nr] = TX(-s);
+    }
+  }
+}
+
+template <typename TW, typename TX>
+__device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int bid) {
+  const int lane = threadIdx.x & 63;
+  const int p = bid * 4 + int(threadIdx.x >> 6);
+  if (p >= a.npairs) return;
+  const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
+  int j, jp, jpp;
+  pair_index(p, a.L, a.N, j, jp, jpp);
+  const size_t nn = size_t(n) * n;
+  const TW* Dj = a.D + size_t(j) * nn;
+  const TW* Ej = a.E + size_t(j) * nn;
+  const TW* Bj = a.B + size_t(j) * n * nx;
+  const TW* Tj = a.Tm + size_t(j) * tsize(n);
+  const bool row = lane < n;
+  const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
+  for (int q = 0; q < nrhs; ++q) {
+    TX* x = a.x + size_t(q) * a.ldx;
+    const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
+    const TX vjpp = row ? x[size_t(jpp) * n + lane] : TX(0);
+    TX s = row ? x[size_t(j) * n + lane] : TX(0);
+    TW d = row ? Dj[lane] : TW(0), e = row ? Ej[lane] : TW(0);
+    for (int c = 0; c < n; ++c) {
+      TW dn = TW(0), en = TW(0);
+      if (c + 1 < n && row) {
+        dn = Dj[lane + size_t(c + 1) * n];
+        en = Ej[lane + size_t(c + 1) * n];
+      }
+      s -= TX(d) * TX(rlane(double(vjp), c)) + TX(e) * TX(rlane(double(vjpp), c));
+      d = dn;
+      e = en;
+    }
+    for (int c = 0; c < nx; ++c) s -= TX(row ? Bj[lane + size_t(c) * n] : TW(0)) * x[xbrow + c];
+    // U^{-1}, bottom up: lane c owns the diagonal of column c
+    TW u = row ? Tj[lane + size_t(n - 1) * n2] : TW(0);
+    for (int c = n - 1; c >= 0; --c) {
+      TW un = TW(0);
+      if (c > 0 && row) un = Tj[lane + size_t(c - 1) * n2];
+      const TX den = lane == c ? TX(u) : TX(1);
+      const TX xc = TX(rlane(double(s / den), c));
+      if (lane < c) s -= TX(u) * xc;
+      else if (lane == c) s = xc;
+      u = un;
+    }
+    if (row) x[size_t(j) * n + lane] = s;
+  }
+}
+
+template <typename TW, typename TX, bool FWD>
+__global__ void __launch_bounds__(256) sweep_lu_stream_kernel(SolveArgs<TW, TX> a) {
+  if (FWD) forward_lu_stream_body<TW, TX>(a, blockIdx.x);
+  else backward_lu_stream_body<TW, TX>(a, blockIdx.x);
+}
+
+template <typename TW, typename TX, bool FWD>
+__global__ void __launch_bounds__(256) sweep_lu_stream_batched(BatchTable<SolveArgs<TW, TX>> t) {
+  const int e = t.entry[blockIdx.x];
+  if (FWD) forward_lu_stream_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
+  else backward_lu_stream_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
+}  // namespace
+
+template <typename TW, typename TX>
+bool launch_sweep_lu_stream(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st, hipError_t* err) {
+  if (a.n < 1 || a.n > 64 || a.npairs < 1) return false;
+  const int blocks = (a.npairs + 3) / 4;
+  if (forward) hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, true>), dim3(blocks), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, false>), dim3(blocks), dim3(256), 0, st, a);
+  *err = hipGetLastError();
+  return true;
+}
+
+template <typename TW, typename TX>
+bool launch_sweep_lu_stream_batched(bool forward, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks, int n,
+                                    hipStream_t st, hipError_t* err) {
+  if (n < 1 || n > 64) return false;
+  if (forward) hipLaunchKernelGGL((sweep_lu_stream_batched<TW, TX, true>), dim3(nblocks), dim3(256), 0, st, t);
+  else hipLaunchKernelGGL((sweep_lu_stream_batched<TW, TX, false>), dim3(nblocks), dim3(256), 0, st, t);
+  *err = hipGetLastError();
+  return true;
+}
+
+#define BCR_INST(TW, TX)                                                                                         \
+  template bool launch_sweep_lu_stream<TW, TX>(bool, SolveArgs<TW, TX> const&, hipStream_t, hipError_t*);       \
+  template bool launch_sweep_lu_stream_batched<TW, TX>(bool, BatchTable<SolveArgs<TW, TX>> const&, int, int,     \
+                                                       hipStream_t, hipError_t*);
+BCR_INST(double, double)
+BCR_INST(float, float)
+BCR_INST(float, double)
+#undef BCR_INST
+
+}  // namespace bcr

@@ -288,6 +288,8 @@ def main():
                          "the learned LU/QRP choice")
     ap.add_argument("--nsys", type=int, default=0)
     ap.add_argument("--cpu-sample-sys", type=int, default=48)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -333,6 +335,12 @@ def main():
             raise RuntimeError(g.last_error())
         g.solve_device(buf, nrhs, L, sync=True)
 
+    if args.calibrate_copy:
+        ca, cb = DeviceArray(1 << 27), DeviceArray(1 << 27)   # 1 GiB each
+        cb.copy_from(ca)
+        device_synchronize()
+        ca.free()
+        cb.free()
     for _ in range(args.warmup):
         step()
     device_synchronize()
@@ -366,14 +374,18 @@ def main():
     B16[nrows:] = 0
     b16 = DeviceArray.from_host(np.asfortranarray(B16).reshape(-1, order="F"))
     X16 = DeviceArray(L * nrhs)
-    for _ in range(2):
-        step((X16, b16), nrhs)
-    k16 = max(3, args.steps // 4)
-    t2 = time.perf_counter()
-    for _ in range(k16):
-        step((X16, b16), nrhs)
-    device_synchronize()
-    ms16 = (time.perf_counter() - t2) * 1e3 / k16
+    try:
+        for _ in range(2):
+            step((X16, b16), nrhs)
+        k16 = max(3, args.steps // 4)
+        t2 = time.perf_counter()
+        for _ in range(k16):
+            step((X16, b16), nrhs)
+        device_synchronize()
+        ms16 = (time.perf_counter() - t2) * 1e3 / k16
+    except Exception as e:   # reported, not fatal: the 1-RHS figure is the metric
+        ms16 = None
+        print(f"[bench] 16-RHS solve failed: {e}", file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: pair elimination, all levels
     pairs_per_factorize = N - 1

