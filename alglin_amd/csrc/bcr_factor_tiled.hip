@@ -22,11 +22,15 @@
 namespace bcr {
 namespace {
 
-constexpr int TR = 32, TC = 16, MR = 5, MC = 13;   // R <= 160, Wc <= 208
-constexpr int RMAX = TR * MR, WMAX = TC * MC;
+constexpr int TR = 32, TC = 16;
+// tile depths: MR x MC values per thread cover R <= 32 MR rows and Wc <= 16 MC
+// columns; (5, 13) serves every n <= 64, (3, 7) the n <= 32 panels with fewer
+// registers (more workgroups per CU)
+constexpr int kBigMR = 5, kBigMC = 13, kSmallMR = 3, kSmallMC = 7;
 
-template <typename TS, typename TW>
+template <typename TS, typename TW, int MR, int MC>
 __device__ __forceinline__ void pair_lu_tiled_body(FactorArgs<TS, TW> a, int bid) {
+  constexpr int RMAX = TR * MR, WMAX = TC * MC;
   __shared__ TW colbuf2[2][RMAX];   // column c in slot c & 1: a thread may still read step c's
                                     // while the owners of column c+1 write theirs
   __shared__ TW rowbuf[WMAX];
@@ -131,6 +135,8 @@ __device__ __forceinline__ void pair_lu_tiled_body(FactorArgs<TS, TW> a, int bid
       }
     }
     __syncthreads();
+    // the pivot row's trailing part, zero elsewhere (columns <= c and the
+    // padding), so the update below needs no per-element test
     const int prow = s_piv;
     if (tr == prow % TR) {
       const int ip = prow / TR;
@@ -140,23 +146,31 @@ __device__ __forceinline__ void pair_lu_tiled_body(FactorArgs<TS, TW> a, int bid
         TW x = TW(0);
 #pragma unroll
         for (int i = 0; i < MR; ++i) x = (i == ip) ? v[i][k] : x;
-        if (s > c && s < Wc) rowbuf[s] = x;
+        rowbuf[s] = (s > c && s < Wc) ? x : TW(0);
       }
     }
     __syncthreads();
     const TW pv = colbuf[prow];
     const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
+    TW lv[MR], rb[MC];
 #pragma unroll
-    for (int i = 0; i < MR; ++i) {
+    for (int i = 0; i < MR; ++i) {   // multiplier of each row, zero for rows not updated
       const int r = tr + TR * i;
-      if (r < R && spos[r] > c) {
-        const TW l = colbuf[r] * rinv;
+      lv[i] = (r < R && spos[r] > c) ? colbuf[r] * rinv : TW(0);
+    }
 #pragma unroll
-        for (int k = 0; k < MC; ++k) {
-          const int s = tc + TC * k;
-          if (s == c) v[i][k] = l;
-          else if (s > c && s < Wc) v[i][k] = fma(-l, rowbuf[s], v[i][k]);
-        }
+    for (int k = 0; k < MC; ++k) rb[k] = rowbuf[tc + TC * k];
+#pragma unroll
+    for (int i = 0; i < MR; ++i)
+#pragma unroll
+      for (int k = 0; k < MC; ++k) v[i][k] = fma(-lv[i], rb[k], v[i][k]);
+    if (tc == c % TC) {   // column c becomes the multipliers
+#pragma unroll
+      for (int i = 0; i < MR; ++i) {
+        const int r = tr + TR * i;
+        const bool u = r < R && spos[r] > c;
+#pragma unroll
+        for (int k = 0; k < MC; ++k) v[i][k] = (u && k == kc) ? lv[i] : v[i][k];
       }
     }
     // rowbuf is rewritten next step only after its second barrier
@@ -195,25 +209,31 @@ __device__ __forceinline__ void pair_lu_tiled_body(FactorArgs<TS, TW> a, int bid
   for (int c = tid; c < n; c += blockDim.x) a.perm[size_t(j) * n + c] = s_ipiv[c];
 }
 
-template <typename TS, typename TW>
+template <typename TS, typename TW, int MR, int MC>
 __global__ void __launch_bounds__(512) pair_lu_tiled_kernel(FactorArgs<TS, TW> a) {
-  pair_lu_tiled_body<TS, TW>(a, blockIdx.x);
+  pair_lu_tiled_body<TS, TW, MR, MC>(a, blockIdx.x);
 }
 
-template <typename TS, typename TW>
+template <typename TS, typename TW, int MR, int MC>
 __global__ void __launch_bounds__(512) pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> t) {
   const int e = t.entry[blockIdx.x];
-  pair_lu_tiled_body<TS, TW>(t.args[e], int(blockIdx.x) - t.first[e]);
+  pair_lu_tiled_body<TS, TW, MR, MC>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
+
+bool fits(int n, int nr, int nx, int mr, int mc) { return n <= 64 && 2 * n + nr <= TR * mr && 3 * n + nx <= TC * mc; }
 
 }  // namespace
 
-bool pair_lu_tiled_fits(int n, int nr, int nx) { return n <= 64 && 2 * n + nr <= RMAX && 3 * n + nx <= WMAX; }
+bool pair_lu_tiled_fits(int n, int nr, int nx) { return fits(n, nr, nx, kBigMR, kBigMC); }
 
 template <typename TS, typename TW>
 bool launch_pair_lu_tiled(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err) {
-  if (!pair_lu_tiled_fits(a.n, a.nr, a.nx)) return false;
-  hipLaunchKernelGGL((pair_lu_tiled_kernel<TS, TW>), dim3(a.npairs), dim3(512), 0, st, a);
+  if (fits(a.n, a.nr, a.nx, kSmallMR, kSmallMC))
+    hipLaunchKernelGGL((pair_lu_tiled_kernel<TS, TW, kSmallMR, kSmallMC>), dim3(a.npairs), dim3(512), 0, st, a);
+  else if (fits(a.n, a.nr, a.nx, kBigMR, kBigMC))
+    hipLaunchKernelGGL((pair_lu_tiled_kernel<TS, TW, kBigMR, kBigMC>), dim3(a.npairs), dim3(512), 0, st, a);
+  else
+    return false;
   *err = hipGetLastError();
   return true;
 }
@@ -221,8 +241,12 @@ bool launch_pair_lu_tiled(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_
 template <typename TS, typename TW>
 bool launch_pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                   hipStream_t st, hipError_t* err) {
-  if (!pair_lu_tiled_fits(n, nr, nx)) return false;
-  hipLaunchKernelGGL((pair_lu_tiled_batched<TS, TW>), dim3(nblocks), dim3(512), 0, st, t);
+  if (fits(n, nr, nx, kSmallMR, kSmallMC))
+    hipLaunchKernelGGL((pair_lu_tiled_batched<TS, TW, kSmallMR, kSmallMC>), dim3(nblocks), dim3(512), 0, st, t);
+  else if (fits(n, nr, nx, kBigMR, kBigMC))
+    hipLaunchKernelGGL((pair_lu_tiled_batched<TS, TW, kBigMR, kBigMC>), dim3(nblocks), dim3(512), 0, st, t);
+  else
+    return false;
   *err = hipGetLastError();
   return true;
 }

@@ -288,6 +288,8 @@ def main():
                          "the learned LU/QRP choice")
     ap.add_argument("--nsys", type=int, default=0)
     ap.add_argument("--cpu-sample-sys", type=int, default=48)
+    ap.add_argument("--pair-kernel", type=int, default=0,
+                    help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()
@@ -322,6 +324,7 @@ def main():
     g.select("LU", "LU", "NONE")
     if args.mode == "mixed":
         g.select_precision("MIXED", 1e-12, 30)
+    g.set_pair_kernel(args.pair_kernel)
     g.bind_device(dev)
     L = max(nrows, ncols)
     bd = DeviceArray.from_host(np.concatenate([b, np.zeros(L - nrows)]))
