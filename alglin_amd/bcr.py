@@ -270,7 +270,8 @@ class BorderedCR:
         self._check(lib().alglin_bcr_set_precision(self._h, mode.encode(), tol, max_iter), "select_precision")
 
     def set_pair_kernel(self, variant: int):
-        """LU pair kernel: 0 default, 2 two-role kernel (16 < n <= 32), 3 register-tiled for every n."""
+        """LU pair kernel: 0 default, 2 two-role kernel (16 < n <= 32), 3 register tiles for every n,
+        4 column-split waves for every n <= 64."""
         self._check(lib().alglin_bcr_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def set_fallback(self, on: bool):

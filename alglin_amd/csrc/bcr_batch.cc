@@ -350,6 +350,8 @@ void BatchEngine<TA, TW>::factorize_async() {
         bool ok = true;
         if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
         else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
+        else if (l.cls == 4 && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
+          ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
           ok = launch_pair_lu_tiled_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
         else {

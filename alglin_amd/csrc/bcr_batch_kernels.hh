@@ -88,6 +88,9 @@ template <typename TS, typename TW>
 bool launch_pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                   hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
+bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
+                                 hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
 bool launch_pair_lu_tw_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>

@@ -168,10 +168,16 @@ void Engine<TA, TW>::level_factor(int L, bool first) {
     using A = std::remove_reference_t<decltype(a)>;
     (void)sizeof(A);
     hipError_t e = hipSuccess;
-    const bool lu_fast = method_ == kLU && ((pair_variant_ == 2 && launch_pair_lu_tw(a, st_, &e)) ||
-                                            (pair_variant_ != 3 && launch_pair_lu_split(a, st_, &e)) ||
-                                            (pair_variant_ != 3 && launch_pair_lu_fast(a, st_, &e)) ||
-                                            launch_pair_lu_tiled(a, st_, &e));
+    // variants: 0 default (wave-per-row kernels for n <= 32, column-split
+    // waves up to n = 64, register tiles beyond), 2 two-role (16 < n <= 32),
+    // 3 register tiles only, 4 column-split waves only
+    const int v = pair_variant_;
+    const bool lu_fast =
+        method_ == kLU &&
+        ((v == 2 && launch_pair_lu_tw(a, st_, &e)) || (v == 4 && launch_pair_lu_cols(a, st_, &e)) ||
+         (v != 3 && v != 4 &&
+          (launch_pair_lu_split(a, st_, &e) || launch_pair_lu_fast(a, st_, &e) || launch_pair_lu_cols(a, st_, &e))) ||
+         launch_pair_lu_tiled(a, st_, &e));
     if (lu_fast) {
       stats_.fast_path = 1;
       hip_check(e, "pair_lu_fast");

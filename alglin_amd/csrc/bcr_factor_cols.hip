@@ -1,0 +1,302 @@
+// Pair elimination, LU, for block sizes up to 64: the columns of the
+// augmented panel (R = 2n + nr rows, Wc = 3n + nx columns, the algebra of
+// bcr_factor_generic.hip) are dealt out over the NW waves of a workgroup and
+// its rows over the 64 lanes (row lane + 64 s, s < RS), all in registers.
+// Wave w holds the T columns w + NW k (k < KT) and the right-hand columns
+// n + w + NW k (k < KO).
+//
+// Per elimination step c there is one barrier:
+//  * every wave reads column c (published in LDS by its owner) for its lanes'
+//    rows and finds the pivot itself: a DPP max-reduction of |a_rc| over the
+//    rows at positions >= c among the first 2n, ties to the smaller position
+//    (LAPACK idamax order).  The row positions live in registers and are the
+//    same in every wave, so no wave waits for another to pick the pivot;
+//  * the pivot row's entry of each owned column is broadcast from the pivot
+//    lane by v_readlane (every wave holds every row of its columns), so the
+//    pivot row is never published;
+//  * the owner of column c + 1 publishes it, updated, in the other LDS slot
+//    (the step's one barrier follows every wave's updates).  Finished T
+//    columns (< c) are skipped.
+// Rows are not moved: each is stored at its final position at the end.  The
+// arithmetic is the generic kernel's (multiplier = a_rc * (1/pivot), fma
+// update), so the factors agree with it bit for bit.
+#include "bcr_launch.hh"
+#include "bcr_batch_kernels.hh"
+#include "bcr_wave.hh"
+
+namespace bcr {
+namespace {
+
+// one element (r, s) of the augmented panel, from the source blocks
+// (r < R, s < Wc); the pending-C rules are those of the generic kernel
+template <typename TS, typename TW>
+__device__ __forceinline__ TW panel_elem(FactorArgs<TS, TW> const& a, int r, int s, int j, int jp, TW* pend_cur,
+                                         TW* pend_prv) {
+  const int n = a.n, nr = a.nr, n2 = 2 * n;
+  const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * a.nx;
+  if (r < n) {                       // row of block jp: [E_jp | D_jp | 0 | B_jp]
+    if (s < n) return TW(a.sE[jp * nn + r + size_t(s) * n]);
+    if (s < n2) return TW(a.sD[jp * nn + r + size_t(s - n) * n]);
+    if (s >= 3 * n) return TW(a.sB[jp * bn + r + size_t(s - 3 * n) * n]);
+    return TW(0);
+  }
+  if (r < n2) {                      // row of block j: [D_j | 0 | E_j | B_j]
+    const int rr = r - n;
+    if (s < n) return TW(a.sD[j * nn + rr + size_t(s) * n]);
+    if (s >= n2 && s < 3 * n) return TW(a.sE[j * nn + rr + size_t(s - n2) * n]);
+    if (s >= 3 * n) return TW(a.sB[j * bn + rr + size_t(s - 3 * n) * n]);
+    return TW(0);
+  }
+  const int ri = r - n2;             // border row: [C_j | C_jp | 0 | 0]
+  if (s < n) {
+    const size_t off = j * rn + ri + size_t(s) * nr;
+    TW x = TW(a.sC[off]);
+    if (!a.first) x += pend_prv[off] + pend_cur[off];
+    return x;
+  }
+  if (s < n2) {
+    const size_t off = jp * rn + ri + size_t(s - n) * nr;
+    TW x = TW(a.sC[off]);
+    if (!a.first) {
+      x += pend_prv[off];
+      pend_prv[off] = TW(0);   // single reader: clear for reuse
+    }
+    return x;
+  }
+  return TW(0);
+}
+
+// panel element (row at final position q, column s) to its factor block
+template <typename TS, typename TW>
+__device__ __forceinline__ void store_elem(FactorArgs<TS, TW> const& a, int q, int s, TW x, int j, int jp, int jpp,
+                                           TW* Tj, TW* pend_cur) {
+  const int n = a.n, nr = a.nr, n2 = 2 * n;
+  const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * a.nx;
+  if (q < n2) {
+    const bool piv = q < n;               // pivot rows -> block j, reduced rows -> block jp
+    const int rp = piv ? q : q - n;
+    const size_t blk = piv ? size_t(j) : size_t(jp);
+    if (s < n) Tj[q + size_t(s) * n2] = x;
+    else if (s < n2) a.D[blk * nn + rp + size_t(s - n) * n] = x;
+    else if (s < 3 * n) a.E[blk * nn + rp + size_t(s - n2) * n] = x;
+    else a.B[blk * bn + rp + size_t(s - 3 * n) * n] = x;
+  } else {
+    const int ri = q - n2;
+    if (s < n) a.C[j * rn + ri + size_t(s) * nr] = x;
+    else if (s < n2) a.C[jp * rn + ri + size_t(s - n) * nr] = x;
+    else if (s < 3 * n) pend_cur[jpp * rn + ri + size_t(s - n2) * nr] += x;
+    else a.Fpart[size_t(j) * nr * a.nx + ri + size_t(s - 3 * n) * nr] = x;
+  }
+}
+
+// the column updates of step c with the pivot row in row set SP, lane `who`
+template <int NW, int RS, int KT, int KO, typename TW>
+__device__ __forceinline__ void update_columns(TW (&vt)[RS][KT], TW (&vo)[RS][KO], const TW (&lv)[RS],
+                                               const bool (&upd)[RS], bool sp, int who, int w, int c, int n,
+                                               int no, TW* next, int lane) {
+  constexpr int S = RS > 1 ? 1 : 0;
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int col = w + NW * k;
+    if (col > c && col < n) {
+      const TW u = wave::rl(sp ? vt[S][k] : vt[0][k], who);
+#pragma unroll
+      for (int s = 0; s < RS; ++s) vt[s][k] = fma(-lv[s], u, vt[s][k]);
+      if (col == c + 1) {   // the next step's pivot column
+#pragma unroll
+        for (int s = 0; s < RS; ++s) next[lane + 64 * s] = vt[s][k];
+      }
+    } else if (col == c) {   // column c becomes the multipliers
+#pragma unroll
+      for (int s = 0; s < RS; ++s) vt[s][k] = upd[s] ? lv[s] : vt[s][k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KO; ++k) {
+    if (w + NW * k < no) {
+      const TW u = wave::rl(sp ? vo[S][k] : vo[0][k], who);
+#pragma unroll
+      for (int s = 0; s < RS; ++s) vo[s][k] = fma(-lv[s], u, vo[s][k]);
+    }
+    if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <typename TS, typename TW, int NMAX, int NXMAX, int NW, int RS>
+__device__ __forceinline__ void pair_lu_cols_body(FactorArgs<TS, TW> const& a, int p) {
+  constexpr int KT = (NMAX + NW - 1) / NW;              // T columns per wave
+  constexpr int KO = (2 * NMAX + NXMAX + NW - 1) / NW;  // D, E, B columns per wave
+  __shared__ TW colbuf[2][RS * 64];
+  if (p >= a.npairs) return;   // uniform over the workgroup
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int n = a.n, nr = a.nr, nx = a.nx, n2 = 2 * n;
+  const int R = n2 + nr, no = n2 + nx;
+  int j, jp, jpp;
+  pair_index(p, a.L, a.N, j, jp, jpp);
+  const size_t rn = size_t(nr) * n;
+  TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
+  TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;
+
+  // ---- load --------------------------------------------------------------------
+  TW vt[RS][KT], vo[RS][KO];
+#pragma unroll
+  for (int s = 0; s < RS; ++s) {
+    const int r = lane + 64 * s;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int col = w + NW * k;
+      vt[s][k] = (r < R && col < n) ? panel_elem(a, r, col, j, jp, pend_cur, pend_prv) : TW(0);
+    }
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+      const int col = w + NW * k;
+      vo[s][k] = (r < R && col < no) ? panel_elem(a, r, n + col, j, jp, pend_cur, pend_prv) : TW(0);
+    }
+  }
+  if (w == 0) {
+#pragma unroll
+    for (int s = 0; s < RS; ++s) colbuf[0][lane + 64 * s] = vt[s][0];
+  }
+  // every wave has read its sources (some are rewritten at the end) and
+  // column 0 is published
+  __syncthreads();
+
+  // ---- n elimination steps --------------------------------------------------------
+  int pos0 = lane, pos1 = lane + 64;   // positions of rows lane, lane + 64
+  int myperm = 0;
+  bool zero_pivot = false;
+  for (int c = 0; c < n; ++c) {
+    const TW* cur = colbuf[c & 1];
+    TW col[RS];
+#pragma unroll
+    for (int s = 0; s < RS; ++s) col[s] = cur[lane + 64 * s];
+    const bool c0 = lane < n2 && pos0 >= c;
+    const bool c1 = RS > 1 && lane + 64 < n2 && pos1 >= c;
+    unsigned h0, l0, h1 = 0, l1 = 0;
+    wave::abs_key(col[0], h0, l0);
+    if (RS > 1) wave::abs_key(col[RS > 1 ? 1 : 0], h1, l1);
+    // the better of this lane's two candidates (ties to the smaller position)
+    const bool b1 = c1 && (!c0 || h1 > h0 || (h1 == h0 && (l1 > l0 || (l1 == l0 && pos1 < pos0))));
+    bool zero;
+    const int who = wave::argmax_pos(c0 || c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane, zero);
+    const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
+    const int sp = __builtin_amdgcn_readlane(int(b1), who);
+    zero_pivot = zero_pivot || zero;
+    if (lane == c) myperm = P + 1;
+    // the pivot row moves to position c, the row at c to the pivot's old place
+    pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
+    pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+    const TW pv = sp ? wave::rl(col[RS > 1 ? 1 : 0], who) : wave::rl(col[0], who);
+    const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
+    bool upd[RS];
+    TW lv[RS];
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {   // rows below the pivot: multiplier, others 0
+      const int r = lane + 64 * s;
+      upd[s] = r < n2 ? (s == 0 ? pos0 > c : pos1 > c) : r < R;
+      lv[s] = upd[s] ? col[s] * rinv : TW(0);
+    }
+    TW* next = colbuf[(c + 1) & 1];
+    update_columns<NW>(vt, vo, lv, upd, sp != 0, who, w, c, n, no, next, lane);
+    // column c + 1 published; every wave is done reading column c's slot
+    __syncthreads();
+  }
+  if (zero_pivot && w == 0 && lane == 0) atomicOr(a.status, 1);
+
+  // ---- store every row at its final position -----------------------------------
+  TW* Tj = a.Tm + size_t(j) * tsize(n);
+#pragma unroll
+  for (int s = 0; s < RS; ++s) {
+    const int r = lane + 64 * s;
+    if (r >= R) continue;
+    const int q = r < n2 ? (s == 0 ? pos0 : pos1) : r;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int col = w + NW * k;
+      if (col < n) store_elem(a, q, col, vt[s][k], j, jp, jpp, Tj, pend_cur);
+    }
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+      const int col = w + NW * k;
+      if (col < no) store_elem(a, q, n + col, vo[s][k], j, jp, jpp, Tj, pend_cur);
+    }
+  }
+  if (w == 0 && lane < n) a.perm[size_t(j) * n + lane] = myperm;
+}
+
+template <typename TS, typename TW, int NMAX, int NXMAX, int NW, int RS>
+__global__ void __launch_bounds__(64 * NW) pair_lu_cols_kernel(FactorArgs<TS, TW> a) {
+  pair_lu_cols_body<TS, TW, NMAX, NXMAX, NW, RS>(a, blockIdx.x);
+}
+
+template <typename TS, typename TW, int NMAX, int NXMAX, int NW, int RS>
+__global__ void __launch_bounds__(64 * NW) pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> t) {
+  const int e = t.entry[blockIdx.x];
+  pair_lu_cols_body<TS, TW, NMAX, NXMAX, NW, RS>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
+// waves per workgroup: FP64 values take two VGPRs, so twice the waves keep
+// about 80 panel registers per lane in both precisions
+template <typename TW> struct ColsWaves;
+template <> struct ColsWaves<float> { static constexpr int k48 = 4, k64 = 8; };
+template <> struct ColsWaves<double> { static constexpr int k48 = 8, k64 = 16; };
+
+// 1: n <= 48, nx <= 4, 2n + nr <= 128 (two row sets); 2: n <= 64, nx <= 16,
+// 2n + nr <= 192 (three row sets); 0: not covered
+int cols_variant(int n, int nr, int nx) {
+  if (n < 1) return 0;
+  if (n <= 48 && nx <= 4 && 2 * n + nr <= 128) return 1;
+  if (n <= 64 && nx <= 16 && 2 * n + nr <= 192) return 2;
+  return 0;
+}
+
+}  // namespace
+
+bool pair_lu_cols_fits(int n, int nr, int nx) { return cols_variant(n, nr, nx) != 0; }
+
+template <typename TS, typename TW>
+bool launch_pair_lu_cols(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err) {
+  constexpr int W1 = ColsWaves<TW>::k48, W2 = ColsWaves<TW>::k64;
+  switch (cols_variant(a.n, a.nr, a.nx)) {
+    case 1:
+      hipLaunchKernelGGL((pair_lu_cols_kernel<TS, TW, 48, 4, W1, 2>), dim3(a.npairs), dim3(64 * W1), 0, st, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((pair_lu_cols_kernel<TS, TW, 64, 16, W2, 3>), dim3(a.npairs), dim3(64 * W2), 0, st, a);
+      break;
+    default:
+      return false;
+  }
+  *err = hipGetLastError();
+  return true;
+}
+
+template <typename TS, typename TW>
+bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
+                                 hipStream_t st, hipError_t* err) {
+  constexpr int W1 = ColsWaves<TW>::k48, W2 = ColsWaves<TW>::k64;
+  switch (cols_variant(n, nr, nx)) {
+    case 1:
+      hipLaunchKernelGGL((pair_lu_cols_batched<TS, TW, 48, 4, W1, 2>), dim3(nblocks), dim3(64 * W1), 0, st, t);
+      break;
+    case 2:
+      hipLaunchKernelGGL((pair_lu_cols_batched<TS, TW, 64, 16, W2, 3>), dim3(nblocks), dim3(64 * W2), 0, st, t);
+      break;
+    default:
+      return false;
+  }
+  *err = hipGetLastError();
+  return true;
+}
+
+#define BCR_INST(TS, TW)                                                                                     \
+  template bool launch_pair_lu_cols<TS, TW>(FactorArgs<TS, TW> const&, hipStream_t, hipError_t*);           \
+  template bool launch_pair_lu_cols_batched<TS, TW>(BatchTable<FactorArgs<TS, TW>> const&, int, int, int, int, \
+                                                    hipStream_t, hipError_t*);
+BCR_INST(double, double)
+BCR_INST(float, float)
+BCR_INST(double, float)
+#undef BCR_INST
+
+}  // namespace bcr

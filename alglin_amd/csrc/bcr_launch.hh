@@ -22,6 +22,11 @@ bool launch_pair_lu_tw(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* 
 bool pair_lu_tiled_fits(int n, int nr, int nx);
 template <typename TS, typename TW>
 bool launch_pair_lu_tiled(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
+// column-split waves LU for n <= 64 (n <= 48 with nx <= 4 and 2n + nr <= 128,
+// else nx <= 16 and 2n + nr <= 192); false when the shape is not covered
+bool pair_lu_cols_fits(int n, int nr, int nx);
+template <typename TS, typename TW>
+bool launch_pair_lu_cols(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 
 template <typename TW, typename TX>
 hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);

@@ -289,7 +289,8 @@ def main():
     ap.add_argument("--nsys", type=int, default=0)
     ap.add_argument("--cpu-sample-sys", type=int, default=48)
     ap.add_argument("--pair-kernel", type=int, default=0,
-                    help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n")
+                    help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n, "
+                         "4 column-split waves for every n <= 64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()

@@ -112,9 +112,11 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T, int32_t* perm, void* D, void
                            double* last_block, double* pending_sumsq);
 /* 1 when the last factorization used the register-resident LU kernel */
 int alglin_bcr_fast_path(alglin_bcr_t h);
-/* LU pair kernel: 0 (default) register kernels for n <= 32 and the
-   register-tiled kernel above; 2 the two-role (panel wave + columns wave)
-   kernel for 16 < n <= 32; 3 the register-tiled kernel for every n */
+/* LU pair kernel: 0 (default) register kernels for n <= 32, the
+   column-split-waves kernel up to n = 64 and the register-tiled kernel
+   beyond; 2 the two-role (panel wave + columns wave) kernel for
+   16 < n <= 32; 3 the register-tiled kernel for every n; 4 the
+   column-split-waves kernel for every n <= 64 */
 int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant);
 
 /* ---- precision (extension of the reference surface) -------------------------- */

@@ -238,9 +238,10 @@ def test_device_resident_path():
                                    (21, 5, 2, 2, 2, 2), (17, 40, 5, 5, 3, 3), (9, 64, 16, 16, 8, 8)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_pair_kernel_variants_agree(oracle_mod, shape, dtype):
-    """Every LU pair kernel (default, two-role, register-tiled) gives the
-    oracle's pivots and the same solution; the written factor panels agree
-    to rounding (bit for bit where the arithmetic order is the same)."""
+    """Every LU pair kernel (default, two-role, register-tiled, column-split
+    waves) gives the oracle's pivots and the same solution; the register-tiled
+    and column-split kernels do the same arithmetic in the same order, so
+    their factor panels agree bit for bit."""
     N, n, qr, qx, nr, nx = shape
     print(f"case shape={shape} dtype={np.dtype(dtype).name}", flush=True)
     s = systems.gaussian_system(N, n, qr, qx, nr, nx, shift=3.0, seed=N + n)
@@ -254,7 +255,8 @@ def test_pair_kernel_variants_agree(oracle_mod, shape, dtype):
     ok, xo = o.solve(b)
     tol = 1e-11 if dtype == np.float64 else 2e-4
     panels = np.arange(1, N)                              # blocks j that are a pair's pivot block
-    for variant in (0, 2, 3):
+    fac = {}
+    for variant in (0, 2, 3, 4):
         g = gpu_solver(s, "LU", dtype)
         g.set_pair_kernel(variant)
         assert g.factorize(), g.last_error()
@@ -265,3 +267,7 @@ def test_pair_kernel_variants_agree(oracle_mod, shape, dtype):
         f = g.factors()
         perm = f["perm"].reshape(N, n)[panels]
         assert np.array_equal(perm, o.internals()["perm"].reshape(N, n)[panels]), variant
+        fac[variant] = dict(T=f["T"].reshape(N, -1)[panels, :2 * n * n], D=f["D"].reshape(N, -1)[panels],
+                            E=f["E"].reshape(N, -1)[panels])
+    for k in ("T", "D", "E"):
+        assert np.array_equal(fac[3][k], fac[4][k]), k
