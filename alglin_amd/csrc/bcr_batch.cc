@@ -136,6 +136,7 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
   Hm_.resize(one(offHm_[ns]));
   ipiv_.resize(one(offPiv_[ns]));
   status_.resize(size_t(ns));
+  skip_.resize(size_t(ns));
   Xpart_.resize(one(offXpart_[ns]));
   Cpart_.resize(one(offCpart_[ns]));
   Csum_.resize(one(offCsum_[ns]));
@@ -339,6 +340,7 @@ void BatchEngine<TA, TW>::factorize_async() {
   const int ns = int(active_.size());
   if (ns == 0) return;
   hip_check(launch_convert<int32_t, int32_t>(nullptr, status_.p, nsys(), st_), "status");
+  hip_check(launch_convert<int32_t, int32_t>(nullptr, skip_.p, nsys(), st_), "skip flags");
   hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
   hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
   for (int k = 0; k < levels_; ++k)
@@ -402,6 +404,7 @@ typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(dou
         a.D = wD_.p + off_[kbDE][s]; a.E = wE_.p + off_[kbDE][s]; a.B = wB_.p + off_[kbB][s];
         a.C = wC_.p + off_[kbC][s]; a.Tm = Tm_.p + offT_[s]; a.perm = perm_.p + offPerm_[s];
         a.Xpart = Xpart_.p + offXpart_[s];
+        a.skip = skip_.p + s;
         args.push_back(a);
         nmax = std::max(nmax, d.n);
         nxmax = std::max(nxmax, d.nx);
@@ -434,10 +437,12 @@ typename BatchEngine<TA, TW>::SolveTables& BatchEngine<TA, TW>::solve_tables(dou
     r.part = Xpart_.p + offXpart_[s]; r.first = 1; r.count = d.N - 1; r.m = d.nr; r.mrow = std::max(1, d.nr);
     r.ldd = std::max(d.nrows(), d.ncols());
     r.dst = X + off_[kbX][s] + int64_t(d.N + 1) * d.n + d.qr;
+    r.skip = skip_.p + s;
     LastSolveArgs<double>& l = last[i];
     l.N = d.N; l.n = d.n; l.qr = d.qr; l.qx = d.qx; l.nr = d.nr; l.nx = d.nx; l.Nr = d.Nr(); l.Nc = d.Nc();
     l.method = kLastLU; l.nrhs = 1; l.Hm = Hm_.p + offHm_[s]; l.P = ipiv_.p + offPiv_[s]; l.Q = nullptr;
     l.tau = nullptr; l.x = X + off_[kbX][s]; l.ldx = std::max(d.nrows(), d.ncols());
+    l.skip = skip_.p + s;
   }
   const size_t oxb = bb.add(xb), olast = bb.add(last);
   bb.upload(T.blob, st_);
@@ -504,12 +509,14 @@ typename BatchEngine<TA, TW>::MvTables& BatchEngine<TA, TW>::mv_tables(double co
     m.H = pH_ + off_[kbH][s]; m.Cq = pCq_ + off_[kbCq][s]; m.F = pF_ + off_[kbF][s];
     m.x = X + off_[kbX][s]; m.b = B ? B + off_[kbX][s] : nullptr; m.out = R + off_[kbX][s];
     m.Cpart = Cpart_.p + offCpart_[s];
+    m.skip = skip_.p + s;
     const int blocks = (d.N + 1 + 3) / 4;
     for (int b = 0; b < blocks; ++b) entry.push_back(i);
     first.push_back(first.back() + blocks);
     ReduceArgs<double, double>& r = cs[i];
     r.part = Cpart_.p + offCpart_[s]; r.first = 0; r.count = d.N + 1; r.m = d.nr; r.mrow = std::max(1, d.nr);
     r.ldd = r.mrow; r.dst = Csum_.p + offCsum_[s];
+    r.skip = skip_.p + s;
     csp[i] = Csum_.p + offCsum_[s];
   }
   Blob bb;
@@ -550,6 +557,7 @@ const CopyArgs* BatchEngine<TA, TW>::add_table(double const* src, double* dst) {
     c[i].src = src + off_[kbX][s];
     c[i].dst = dst + off_[kbX][s];
     c[i].len = dims_[s].ncols();
+    c[i].skip = skip_.p + s;
   }
   Blob bb;
   const size_t o = bb.add(c);
@@ -587,6 +595,7 @@ void BatchEngine<TA, TW>::norms_async(double const* B, double const* R, double* 
       a[i].r = R + off_[kbX][s];
       a[i].len = dims_[s].nrows();
       a[i].sys = s;
+      a[i].skip = skip_.p + s;
     }
     Blob bb;
     const size_t o = bb.add(a);
@@ -670,6 +679,12 @@ void BatchSolver::plan(std::vector<SysDims> const& dims) {
   dims_ = dims;
   R_.resize(size_t(std::max<int64_t>(total(kbX), 1)));
   norms_.resize(2 * dims.size());
+  drel_.resize(dims.size());
+  dbest_.resize(dims.size());
+  dstat_.resize(dims.size());
+  dsweeps_.resize(dims.size());
+  dstag_.resize(dims.size());
+  dopen_.resize(1);
 }
 
 int64_t BatchSolver::total(int kind) const { return mixed_ ? e32_->total(kind) : e64_->total(kind); }
@@ -717,19 +732,28 @@ hipGraphExec_t capture(hipStream_t st, F&& body) {
 }
 }  // namespace
 
-// Refinement sweeps over the systems not yet done: residual, host test
-// (converged / stalled: three sweeps without halving the best residual /
-// factorization failed), correction solve.  With graphs the residual and
-// correction launch lists are replayed (they cover the engine's active set).
+// Refinement sweeps over the systems not yet done: residual and norms, the
+// convergence test on the device (converged / stalled: three sweeps without
+// halving the best residual / factorization failed), correction solve.  A
+// done system is skipped by every launch through the engine's skip flags, so
+// the captured residual and correction graphs replay unchanged while the set
+// of open systems shrinks; one int (the open count) comes back per sweep.
 template <typename E>
 void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done,
                          BatchReport& rep) {
   const int ns = eng.nsys();
   hipStream_t st = eng.stream();
-  const int64_t nx = eng.total(kbX);
   const bool graph = g_.res && g_.corr;
-  std::vector<double> best(ns, INFINITY), h(2 * size_t(ns));
-  std::vector<int> stagnant(ns, 0);
+  std::vector<int32_t> hd(ns);
+  for (int s = 0; s < ns; ++s) hd[s] = done[s] ? 1 : 0;
+  const size_t i4 = sizeof(int32_t) * size_t(ns), d8 = sizeof(double) * size_t(ns);
+  hip_check(hipMemcpyAsync(eng.skip_flags(), hd.data(), i4, hipMemcpyHostToDevice, st), "done flags");
+  hip_check(hipMemcpyAsync(dstat_.p, status_.data(), i4, hipMemcpyHostToDevice, st), "status");
+  hip_check(hipMemcpyAsync(drel_.p, rel_.data(), d8, hipMemcpyHostToDevice, st), "residuals");
+  hip_check(hipMemcpyAsync(dsweeps_.p, sweeps_.data(), i4, hipMemcpyHostToDevice, st), "sweeps");
+  ConvergeArgs ca{norms_.p, dstat_.p, dbest_.p, dstag_.p, eng.skip_flags(), drel_.p, dsweeps_.p, dopen_.p, tol, 0,
+                  ns};
+  int32_t open = 0;
   for (int it = 0;; ++it) {
     if (graph) {
       hip_check(hipGraphLaunch(g_.res, st), "graph residual");
@@ -737,34 +761,13 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
       eng.residual_async(X, B, R_.p);
       eng.norms_async(B, R_.p, norms_.p);
     }
-    hip_check(hipMemcpyAsync(h.data(), norms_.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st), "norms");
-    hip_check(hipStreamSynchronize(st), "norms sync");
-    int open = 0;
-    for (int s : eng.active()) {
-      if (done[s]) continue;
-      const double nb = h[2 * s] > 0 ? h[2 * s] : 1.0;
-      const double rel = h[2 * s + 1] / nb;
-      rel_[s] = rel;
-      sweeps_[s] = it;
-      if (status_[s] != 0 || !(rel == rel) || rel <= tol) { done[s] = 1; continue; }
-      if (rel > 0.5 * best[s]) {
-        if (++stagnant[s] >= 3) { done[s] = 1; continue; }
-      } else {
-        stagnant[s] = 0;
-      }
-      best[s] = std::min(best[s], rel);
-      ++open;
-    }
+    hip_check(launch_convert<int32_t, int32_t>(nullptr, dopen_.p, 1, st), "open count");
+    ca.it = it;
+    hip_check(launch_converge_batched(ca, st), "convergence test");
+    hip_check(hipMemcpyAsync(&open, dopen_.p, sizeof(int32_t), hipMemcpyDeviceToHost, st), "open count");
+    hip_check(hipStreamSynchronize(st), "open count sync");
     rep.sweeps = std::max(rep.sweeps, it);
     if (open == 0 || it >= max_iter) break;
-    // once most of the swept systems are done, sweep only the open ones (the
-    // launch tables are rebuilt for the smaller set; graphs cover the old set)
-    if (!graph && 2 * open <= int(eng.active().size())) {
-      std::vector<char> mask(ns, 0);
-      for (int s : eng.active())
-        if (!done[s]) mask[s] = 1;
-      eng.set_active(mask);
-    }
     if (graph) {
       hip_check(hipGraphLaunch(g_.corr, st), "graph correction");
     } else {
@@ -772,6 +775,11 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
       eng.add_active_async(R_.p, X);
     }
   }
+  hip_check(hipMemcpyAsync(rel_.data(), drel_.p, d8, hipMemcpyDeviceToHost, st), "residuals");
+  hip_check(hipMemcpyAsync(sweeps_.data(), dsweeps_.p, i4, hipMemcpyDeviceToHost, st), "sweeps");
+  hip_check(hipMemcpyAsync(hd.data(), eng.skip_flags(), i4, hipMemcpyDeviceToHost, st), "done flags");
+  hip_check(hipStreamSynchronize(st), "refinement results");
+  for (int s = 0; s < ns; ++s) done[s] = hd[s] != 0;
 }
 
 template <typename E>

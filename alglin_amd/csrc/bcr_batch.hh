@@ -93,6 +93,9 @@ class BatchEngine {
 
   hipStream_t stream() const { return st_; }
   size_t factor_bytes() const { return factor_bytes_; }
+  // per-system flags (device, nsys): the residual, norms, solve and add
+  // launches skip a system whose flag is nonzero; factorize_async clears them
+  int32_t* skip_flags() { return skip_.p; }
 
  private:
   struct Launch {
@@ -143,7 +146,7 @@ class BatchEngine {
   TA const *pD_ = nullptr, *pE_ = nullptr, *pB_ = nullptr, *pC_ = nullptr, *pH_ = nullptr, *pCq_ = nullptr,
            *pF_ = nullptr;
   DevBuf<TW> wD_, wE_, wB_, wC_, Tm_, Cpend_, Fpart_;
-  DevBuf<int32_t> perm_, ipiv_, status_;
+  DevBuf<int32_t> perm_, ipiv_, status_, skip_;
   DevBuf<double> Feff_, Hm_, Xpart_, Cpart_, Csum_;
 
   // factor tables
@@ -209,6 +212,9 @@ class BatchSolver {
   BatchEngine<double, double>* e64_ = nullptr;
   BatchEngine<double, float>* e32_ = nullptr;
   DevBuf<double> R_, norms_;
+  // device side of the refinement test (per system)
+  DevBuf<double> drel_, dbest_;
+  DevBuf<int32_t> dstat_, dsweeps_, dstag_, dopen_;
   std::vector<int32_t> status_, sweeps_;
   std::vector<double> rel_;
   std::vector<float> feat_;
@@ -217,7 +223,8 @@ class BatchSolver {
   std::vector<SysDims> dims_;
   double const *oD_ = nullptr, *oE_ = nullptr, *oB_ = nullptr, *oC_ = nullptr, *oH_ = nullptr, *oCq_ = nullptr,
                *oF_ = nullptr;
-  // refinement loop over the active systems of `eng` (host convergence test)
+  // refinement loop over the systems of `eng` not marked done (device
+  // convergence test; done systems are skipped by every launch)
   template <typename E>
   void refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done, BatchReport& rep);
   // captured graphs: factor+solve, residual+norms, correction

@@ -21,6 +21,7 @@ struct ReduceArgs {
   int first, count, m, mrow;
   int64_t ldd;
   TD* dst;
+  const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
 };
 
 struct NormArgs {
@@ -28,6 +29,7 @@ struct NormArgs {
   const double* r;
   int64_t len;
   int sys;   // output slot: out[2*sys], out[2*sys+1]
+  const int32_t* skip = nullptr;   // the system is done when *skip != 0
 };
 
 // synthetic system s: D_k = shift I + G/sqrt(n), E_k = G/sqrt(n), B, C, H, Cq, F = G;
@@ -55,6 +57,7 @@ struct CopyArgs {
   const double* src;
   double* dst;
   int64_t len;
+  const int32_t* skip = nullptr;   // the system is done when *skip != 0
 };
 
 template <typename TW>
@@ -62,6 +65,26 @@ hipError_t launch_features_batched(const FeatArgs<TW>* args, int nsys, float* ou
 hipError_t launch_copy_batched(const CopyArgs* args, int nsys, hipStream_t st);
 // dst_s += src_s per record (the same CopyArgs records)
 hipError_t launch_add_batched(const CopyArgs* args, int nsys, hipStream_t st);
+
+// One refinement test per system, on the device: rel = ||r|| / ||b|| from
+// norms[2s], norms[2s+1]; a system not yet done becomes done when it has
+// converged (rel <= tol), is not finite, its factorization failed
+// (status != 0) or it stalled (three sweeps in a row without halving its best
+// residual); *open counts the systems left open (cleared by the caller).
+struct ConvergeArgs {
+  const double* norms;
+  const int32_t* status;
+  double* best;
+  int32_t* stagnant;
+  int32_t* done;
+  double* rel;
+  int32_t* sweeps;
+  int32_t* open;
+  double tol;
+  int it;      // sweep index; 0 resets best / stagnant
+  int nsys;
+};
+hipError_t launch_converge_batched(ConvergeArgs const& a, hipStream_t st);
 
 template <typename TA, typename TW>
 hipError_t launch_prep_batched(const PrepArgs<TA, TW>* args, int nsys, hipStream_t st);

@@ -34,6 +34,7 @@ template <typename TP, typename TD>
 __global__ void __launch_bounds__(256) reduce_batched(const ReduceArgs<TP, TD>* args) {
   __shared__ double red[256];
   const ReduceArgs<TP, TD>& a = args[blockIdx.x];
+  if (skipped(a.skip)) return;
   for (int i = 0; i < a.m; ++i) {
     double s = 0;
     for (int k = threadIdx.x; k < a.count; k += blockDim.x) s += double(a.part[size_t(a.first + k) * a.m + i]);
@@ -54,6 +55,7 @@ __global__ void __launch_bounds__(256) reduce_batched(const ReduceArgs<TP, TD>* 
 __global__ void __launch_bounds__(256) norms_batched(const NormArgs* args, double* out) {
   __shared__ double red[2][256];
   const NormArgs& a = args[blockIdx.x];
+  if (skipped(a.skip)) return;
   double sb = 0, sr = 0;
   for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) {
     const double vb = a.b[i], vr = a.r[i];
@@ -200,8 +202,36 @@ __global__ void __launch_bounds__(256) copy_batched(const CopyArgs* args) {
   for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) a.dst[i] = a.src[i];
 }
 
+__global__ void __launch_bounds__(256) converge_batched(ConvergeArgs a) {
+  const int s = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= a.nsys || a.done[s]) return;
+  const double nb = a.norms[2 * s] > 0 ? a.norms[2 * s] : 1.0;
+  const double rel = a.norms[2 * s + 1] / nb;
+  a.rel[s] = rel;
+  a.sweeps[s] = a.it;
+  if (a.status[s] != 0 || !(rel == rel) || rel <= a.tol) {
+    a.done[s] = 1;
+    return;
+  }
+  const double best = a.it == 0 ? INFINITY : a.best[s];
+  int stag = a.it == 0 ? 0 : a.stagnant[s];
+  if (rel > 0.5 * best) {
+    if (++stag >= 3) {
+      a.stagnant[s] = stag;
+      a.done[s] = 1;
+      return;
+    }
+  } else {
+    stag = 0;
+  }
+  a.best[s] = fmin(best, rel);
+  a.stagnant[s] = stag;
+  atomicAdd(a.open, 1);
+}
+
 __global__ void __launch_bounds__(256) add_batched(const CopyArgs* args) {
   const CopyArgs& a = args[blockIdx.x];
+  if (skipped(a.skip)) return;
   for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) a.dst[i] += a.src[i];
 }
 
@@ -220,6 +250,12 @@ hipError_t launch_copy_batched(const CopyArgs* args, int nsys, hipStream_t st) {
 
 hipError_t launch_add_batched(const CopyArgs* args, int nsys, hipStream_t st) {
   hipLaunchKernelGGL(add_batched, dim3(nsys), dim3(256), 0, st, args);
+  return hipGetLastError();
+}
+
+hipError_t launch_converge_batched(ConvergeArgs const& a, hipStream_t st) {
+  if (a.nsys < 1) return hipSuccess;
+  hipLaunchKernelGGL(converge_batched, dim3((a.nsys + 255) / 256), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

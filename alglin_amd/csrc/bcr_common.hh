@@ -55,6 +55,7 @@ struct SolveArgs {
   const TW* D; const TW* E; const TW* B; const TW* C;
   const TW* Tm; const int32_t* perm;
   TX* Xpart;          // [N][nr*nrhs]
+  const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
 };
 
 // Arguments of the last-block assembly.
@@ -97,6 +98,7 @@ struct LastSolveArgs {
   // second solver replaced a failed first factorization)
   const int32_t* status;
   const double *U, *V, *S; const int32_t* rank;
+  const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
 };
 
 // Arguments of the FP64 block SpMV (pristine operator).
@@ -108,12 +110,33 @@ struct MvArgs {
   const double* b;     // nrows (residual mode) or nullptr (y = A x)
   double* out;         // nrows
   double* Cpart;       // [N+1][nr]
+  const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
 };
 
 __host__ __device__ __forceinline__ size_t tsize(int n) { return size_t(2) * n * n + n; }
 __host__ __device__ __forceinline__ size_t align16(size_t bytes) { return (bytes + 15) & ~size_t(15); }
 
 template <typename T> __device__ __forceinline__ T dev_abs(T v) { return v < T(0) ? -v : v; }
+
+// v where keep, +0 elsewhere, through an integer mask the compiler cannot see
+// through.  A plain `keep ? v : 0` over a loaded v is turned into a branch
+// around the load (the load sunk into it), which costs one full memory
+// latency per element; with the mask every load is issued unconditionally
+// (from a clamped, valid address) and all of them are in flight together.
+__device__ __forceinline__ float keep_if(float v, bool keep) {
+  int m = keep ? -1 : 0;
+  asm volatile("" : "+v"(m));
+  return __int_as_float(__float_as_int(v) & m);
+}
+__device__ __forceinline__ double keep_if(double v, bool keep) {
+  int m = keep ? -1 : 0;
+  asm volatile("" : "+v"(m));
+  return __longlong_as_double(__double_as_longlong(v) & static_cast<long long>(m));
+}
+
+// batched refinement: a record whose system has converged (or stalled) is
+// skipped by every kernel of the residual and correction launches
+__device__ __forceinline__ bool skipped(const int32_t* f) { return f != nullptr && *f != 0; }
 
 // pair p of level L (pure cyclic reduction over [0,N), cc:1088-1133)
 // Batched launches: workgroup b serves entry e = entry[b] (one system at one

@@ -280,6 +280,7 @@ __global__ void __launch_bounds__(1024) lu_last_batched(const LastSolveArgs<doub
 
 template <typename TX, bool A_LDS>
 __global__ void __launch_bounds__(1024) last_solve_batched(const LastSolveArgs<TX>* args) {
+  if (skipped(args[blockIdx.x].skip)) return;
   last_solve_body<TX, A_LDS>(args[blockIdx.x]);
 }
 

@@ -88,12 +88,14 @@ __global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double
 template <typename TA>
 __global__ void __launch_bounds__(256) mv_rows_batched(BatchTable<MvArgs<TA>> t) {
   const int e = t.entry[blockIdx.x];
+  if (skipped(t.args[e].skip)) return;
   mv_rows_body<TA>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
 // one workgroup per system; Csum[s] = the system's summed border partials
 template <typename TA>
 __global__ void __launch_bounds__(256) mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum) {
+  if (skipped(args[blockIdx.x].skip)) return;
   mv_tail_body<TA>(args[blockIdx.x], Csum[blockIdx.x]);
 }
 

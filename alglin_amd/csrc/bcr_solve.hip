@@ -238,6 +238,7 @@ hipError_t launch_backward(int method, SolveArgs<TW, TX> const& a, hipStream_t s
 template <typename TW, typename TX, int METHOD, bool FWD>
 __global__ void __launch_bounds__(128) sweep_generic_batched(BatchTable<SolveArgs<TW, TX>> t) {
   const int e = t.entry[blockIdx.x];
+  if (skipped(t.args[e].skip)) return;
   if (FWD) forward_body<TW, TX, METHOD>(t.args[e], int(blockIdx.x) - t.first[e]);
   else backward_body<TW, TX, METHOD>(t.args[e], int(blockIdx.x) - t.first[e]);
 }

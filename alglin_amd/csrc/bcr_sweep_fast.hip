@@ -56,10 +56,17 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
   const bool row = live && r < n2;
 
   // row r of the 2n x n panel [L; M] (column-major, ld 2n)
+  // loads are unconditional (clamped to valid addresses, masked after) so
+  // that they are all in flight together instead of one branch and one
+  // memory latency each
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
-  TX t[NMAX];
+  const int rc = row ? r : 0;
+  TW t[NMAX];   // kept in the factor precision, widened where used
 #pragma unroll
-  for (int c = 0; c < NMAX; ++c) t[c] = (row && c < n) ? TX(Tj[r + size_t(c) * n2]) : TX(0);
+  for (int c = 0; c < NMAX; ++c) {
+    const TW v = Tj[rc + size_t(c < n ? c : 0) * n2];
+    t[c] = keep_if(v, row && c < n);
+  }
 
   // row interchanges (laswp over rows 1..n with 1-based ipiv, cc:939):
   // after the swaps row r holds the original row src
@@ -81,30 +88,47 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
     TX x[QB];
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-      x[q] = (row && q0 + q < nrhs) ? a.x[in_row + size_t(q0 + q) * ldx] : TX(0);
-      x[q] = __shfl(x[q], base + src, 64);
+      const int qq = q0 + q < nrhs ? q0 + q : 0;
+      const TX v = a.x[(row ? in_row : size_t(0)) + size_t(qq) * ldx];
+      x[q] = keep_if(v, row && q0 + q < nrhs);
     }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) x[q] = __shfl(x[q], base + src, 64);
     // W <- [L 0; M I]^{-1} W : column c of the panel below the diagonal
 #pragma unroll
     for (int c = 0; c < NMAX; ++c) {
       if (c < n) {
+        TW tc = t[c];
+        asm volatile("" : "+v"(tc));
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
           const TX xc = gbcast<G>(x[q], base, c);
-          x[q] = (r > c) ? x[q] - t[c] * xc : x[q];
+          x[q] = (r > c) ? x[q] - TX(tc) * xc : x[q];
         }
       }
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q)
       if (row && q0 + q < nrhs) a.x[out_row + size_t(q0 + q) * ldx] = x[q];
-    // border partial -C_j' x_j over the top n rows of the group
-    for (int i = 0; i < nr; ++i) {
-      const TX ci = (live && r < n) ? TX(Cj[i + size_t(r) * nr]) : TX(0);
+    // border partial -C_j' x_j over the top n rows of the group, four border
+    // rows' loads in flight at a time
+    const bool crow = live && r < n;
+    for (int i0 = 0; i0 < nr; i0 += 4) {
+      TX ci[4];
 #pragma unroll
-      for (int q = 0; q < QB; ++q) {
-        const TX v = group_sum<G>(ci * x[q]);
-        if (live && r == 0 && q0 + q < nrhs) a.Xpart[size_t(j) * nr * nrhs + i + size_t(q0 + q) * nr] = -v;
+      for (int i = 0; i < 4; ++i) {
+        const int ic = i0 + i < nr ? i0 + i : 0;
+        const TW v = Cj[ic + size_t(crow ? r : 0) * nr];
+        ci[i] = TX(keep_if(v, crow && i0 + i < nr));
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i0 + i >= nr) break;
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+          const TX v = group_sum<G>(ci[i] * x[q]);
+          if (live && r == 0 && q0 + q < nrhs) a.Xpart[size_t(j) * nr * nrhs + i0 + i + size_t(q0 + q) * nr] = -v;
+        }
       }
     }
   }
@@ -134,27 +158,42 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
   // half 0: row rr of D_j and of U; half 1: row rr of E_j
   const TW* M = (h == 0 ? a.D : a.E) + size_t(j) * nn;
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
-  TX m[NMAX], u[NMAX];
+  // loads are unconditional (clamped to valid addresses, masked after) so
+  // that they are all in flight together
+  // (kept in the factor precision, widened where used)
+  const int rc = row ? rr : 0;
+  TW m[NMAX], u[NMAX];
 #pragma unroll
   for (int c = 0; c < NMAX; ++c) {
-    m[c] = (row && c < n) ? TX(M[rr + size_t(c) * n]) : TX(0);
-    u[c] = (row && h == 0 && c < n && c >= rr) ? TX(Tj[rr + size_t(c) * n2]) : TX(0);
+    const int cc = c < n ? c : 0;
+    const TW vm = M[rc + size_t(cc) * n];
+    const TW vu = Tj[rc + size_t(cc) * n2];
+    m[c] = keep_if(vm, row && c < n);
+    u[c] = keep_if(vu, row && h == 0 && c < n && c >= rr);
   }
   const TW* Bj = a.B + size_t(j) * n * nx;
   const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
   const size_t vin_row = size_t(h == 0 ? jp : jpp) * n + rr;
 
   for (int q0 = 0; q0 < nrhs; q0 += QB) {
-    TX s[QB], vin[QB];
+    TX s[QB], vin[QB], xj[QB];
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
-      vin[q] = (row && q0 + q < nrhs) ? a.x[vin_row + size_t(q0 + q) * ldx] : TX(0);
+      const int qq = q0 + q < nrhs ? q0 + q : 0;
+      const TX v = a.x[(row ? vin_row : size_t(0)) + size_t(qq) * ldx];
+      const TX w = a.x[(row ? size_t(j) * n + rr : size_t(0)) + size_t(qq) * ldx];
+      vin[q] = keep_if(v, row && q0 + q < nrhs);
+      xj[q] = keep_if(w, row && h == 0 && q0 + q < nrhs);
       s[q] = TX(0);
     }
-    // half 0: D_j x_jp, half 1: E_j x_jpp
+    // half 0: D_j x_jp, half 1: E_j x_jpp (each factor widened at its use:
+    // the opaque copy keeps the widened values from being hoisted out of the
+    // right-hand-side loop, which would double their registers)
 #pragma unroll
     for (int c = 0; c < NMAX; ++c) {
       if (c < n) {
+        TW mc = m[c];
+        asm volatile("" : "+v"(mc));
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
           TX v;
@@ -164,34 +203,48 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
           } else {
             v = __shfl(vin[q], base + h * NMAX + c, 64);
           }
-          s[q] += m[c] * v;
+          s[q] += TX(mc) * v;
         }
       }
     }
-    // half 0: + B_j xb
-    for (int c = 0; c < nx; ++c) {
-      const TX b = (row && h == 0) ? TX(Bj[rr + size_t(c) * n]) : TX(0);
+    // half 0: + B_j xb, four columns' loads in flight at a time
+    for (int c0 = 0; c0 < nx; c0 += 4) {
+      TX b[4], xb[4][QB];
 #pragma unroll
-      for (int q = 0; q < QB; ++q) {
-        const TX xb = (q0 + q < nrhs) ? a.x[xbrow + c + size_t(q0 + q) * ldx] : TX(0);
-        s[q] += b * xb;
+      for (int c = 0; c < 4; ++c) {
+        const int cc = c0 + c < nx ? c0 + c : 0;
+        const TW v = Bj[rc + size_t(cc) * n];
+        b[c] = TX(keep_if(v, row && h == 0 && c0 + c < nx));
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+          const int qq = q0 + q < nrhs ? q0 + q : 0;
+          const TX w = a.x[xbrow + cc + size_t(qq) * ldx];
+          xb[c][q] = keep_if(w, q0 + q < nrhs);
+        }
       }
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c0 + c < nx) {
+#pragma unroll
+          for (int q = 0; q < QB; ++q) s[q] += b[c] * xb[c][q];
+        }
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const TX other = __shfl(s[q], base + NMAX + rr, 64);
-      const TX xj = (row && h == 0 && q0 + q < nrhs) ? a.x[size_t(j) * n + rr + size_t(q0 + q) * ldx] : TX(0);
-      s[q] = xj - s[q] - other;
+      s[q] = xj[q] - s[q] - other;
     }
     // U^{-1}: column c, bottom up; lane c of half 0 owns the diagonal
 #pragma unroll
     for (int c = NMAX - 1; c >= 0; --c) {
       if (c < n) {
-        const TX den = rr == c ? u[c] : TX(1);
+        TW uc = u[c];
+        asm volatile("" : "+v"(uc));
+        const TX den = rr == c ? TX(uc) : TX(1);
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
           const TX xc = gbcast<G>(s[q] / den, base, c);
-          s[q] = (rr < c) ? s[q] - u[c] * xc : ((rr == c) ? xc : s[q]);
+          s[q] = (rr < c) ? s[q] - TX(uc) * xc : ((rr == c) ? xc : s[q]);
         }
       }
     }
@@ -210,6 +263,7 @@ __global__ void __launch_bounds__(256) backward_lu_fast_kernel(SolveArgs<TW, TX>
 template <typename TW, typename TX, int NMAX, int QB, bool FWD>
 __global__ void __launch_bounds__(256) sweep_lu_fast_batched(BatchTable<SolveArgs<TW, TX>> t) {
   const int e = t.entry[blockIdx.x];
+  if (skipped(t.args[e].skip)) return;
   if (FWD) forward_lu_fast_body<TW, TX, NMAX, QB>(t.args[e], int(blockIdx.x) - t.first[e]);
   else backward_lu_fast_body<TW, TX, NMAX, QB>(t.args[e], int(blockIdx.x) - t.first[e]);
 }

@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(256) sweep_lu_stream_kernel(SolveArgs<TW, TX> 
 template <typename TW, typename TX, bool FWD>
 __global__ void __launch_bounds__(256) sweep_lu_stream_batched(BatchTable<SolveArgs<TW, TX>> t) {
   const int e = t.entry[blockIdx.x];
+  if (skipped(t.args[e].skip)) return;
   if (FWD) forward_lu_stream_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
   else backward_lu_stream_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
 }

@@ -1,26 +1,27 @@
 """Benchmark of the MI355X bordered cyclic-reduction BABD solver.
 
-Default workload: the large single system solved in plain FP64 on one GPU,
-N = 100000 blocks, n = 32, qr = qx = 8, nr = nx = 4, Gaussian blocks with
-diagonal shift 3 (D_k = 3I + G/sqrt(n), E_k = G/sqrt(n), borders N(0,1)),
-b = A * ones.  One step = factorize + solve (1 RHS) of that system with the
-operator already resident in HBM (the rhs copy into the in-place solution
-buffer is inside the timed region).  value = solves per second.
+Default workload (--workload batched): 4096 uneven BABD systems in one
+batched solve, n uniform in 4..48, N log-uniform in [16, 4096],
+qr = qx = n/4, nr = nx = 2, Gaussian blocks with diagonal shift 4, generated
+on the GPU, random right-hand sides.  One step = the whole batch solved to a
+relative residual of 1e-12 per system: FP32 cyclic-reduction factors, FP64
+iterative refinement, the factor/solve, residual and correction launch lists
+replayed as HIP graphs.  value = systems solved per second.  Reported beside
+it: the same batch in plain FP64 through the same batched engine, and
+through the plain FP64 BorderedCR one system at a time (the reference's call
+sequence, timed on a sample and scaled); the per-system sweep counts; a CPU
+baseline (the oracle on a sample of the systems, one system per host thread).
 
-Also reported: the 16-RHS solve time, the FP64 relative residual
-||b - A x|| / ||b|| and the forward error against the exact solution (ones),
-the per-phase GPU time from HIP events on the solver's stream, the roofline
-figure of the dominant kernel (the pair-elimination kernel), and a CPU
-baseline: the oracle's threaded restatement of the reference algorithm
-(range partition over all host threads) timed on a bounded sample of the
-same workload on this machine's host cores.
+Other workloads:
+  --workload selector  8192 systems, half nearly singular, the learned LU/QRP
+                       selector with the FP64 QRP fallback, against the mixed
+                       solve without the selector and against plain FP64;
+  --workload single    one large system (N = 100000, n = 32, qr = qx = 8,
+                       nr = nx = 4, shift 3, b = A * ones) in plain FP64 or
+                       --mode mixed, with the pair-elimination roofline and
+                       the 16-RHS time.
 
-Other workloads (--workload batched | selector): thousands of uneven systems
-in one batched solve (FP32 factors + FP64 refinement, graph replay), and the
-learned LU/QRP selector on a mix of regular and nearly singular systems;
-both against plain FP64 on the same batch.
-
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload single]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload batched]
 """
 from __future__ import annotations
 
@@ -152,6 +153,57 @@ def run_cpu_baseline_batch(bt, dims, sample, reps=2):
                 seconds_sample=t)
 
 
+def time_plain_per_system(bt, dims, B, sample, seed):
+    """The plain FP64 GPU BorderedCR as a drop-in user runs it on a batch: one
+    solver object per system, the reference's call sequence (factorize, then
+    solve) system after system, the operator and right-hand side resident in
+    HBM (each solver is bound to its system's slices of the batch arrays).
+    Timed on a random sample of the systems and scaled to the whole batch."""
+    import ctypes
+    from alglin_amd.bcr import BorderedCR, DeviceArray, device_synchronize, lib
+
+    class _Slice:   # a system's slice of a packed batch array
+        def __init__(self, ptr):
+            self.ptr = ptr
+
+        def data_ptr(self):
+            return self.ptr
+
+    kinds = dict(D="DE", E="DE", B="B", C="C", H="H", Cq="Cq", F="F")
+    pick = np.sort(np.random.default_rng(seed).choice(len(dims), size=min(sample, len(dims)), replace=False))
+    X = DeviceArray(bt.size("x"))
+    solvers = []
+    for s in pick:
+        N, n, qr, qx, nr, nx, _ = (int(v) for v in dims[s])
+        g = BorderedCR()
+        g.allocate(N, n, qr, qx, nr, nx)
+        g.select("LU", "LU", "NONE")
+        g.bind_device({k: _Slice(bt.ops[k].data_ptr() + 8 * bt.offset(kd, s)) for k, kd in kinds.items()})
+        o = 8 * bt.offset("x", s)
+        L = max(2 * n + qr + nr + n * (N - 1), 2 * n + qx + nx + n * (N - 1))
+        solvers.append((g, _Slice(X.data_ptr() + o), B.data_ptr() + o, L))
+
+    def one_pass():
+        for g, x, b, L in solvers:
+            lib().alglin_memcpy_dtod_async(ctypes.c_void_p(x.ptr), ctypes.c_void_p(b), 8 * L,
+                                           ctypes.c_void_p(g.stream()))
+            if not g.factorize_device(sync=False):
+                raise RuntimeError(g.last_error())
+            g.solve_device(x, 1, L, sync=True)
+
+    one_pass()
+    device_synchronize()
+    t0 = time.perf_counter()
+    one_pass()
+    device_synchronize()
+    t = time.perf_counter() - t0
+    scale = len(dims) / len(pick)
+    X.free()
+    return dict(solves_per_s=len(pick) / t, ms=1e3 * t * scale,
+                sample=f"{len(pick)} random systems of the batch, one BorderedCR each, factorize + solve in turn, "
+                       f"{1e3 * t:.1f} ms, scaled x{scale:.1f}")
+
+
 def bench_batched(args):
     """Batched uneven systems: plain FP64 cyclic reduction vs FP32 factors +
     FP64 refinement (graph replay), or, for the selector workload, vs the
@@ -212,6 +264,10 @@ def bench_batched(args):
     else:
         tmix = timed(lambda: mix.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
         rmix = mix.report()
+    try:
+        per_sys = time_plain_per_system(ref, dims, B, args.per_system_sample, args.seed)
+    except Exception as e:   # reported, not fatal
+        per_sys = {"error": str(e)}
     sweeps = int(rmix["max_sweeps"])
     fac = batch_pair_bytes(dims, 4)
     sw, res = batch_sweep_bytes(dims, 4)
@@ -240,8 +296,11 @@ def bench_batched(args):
             "nsys": nsys, "tol": tol, "parallelism": "single-gpu",
         },
         "plain_fp64": dict(solves_per_s=nsys / t64, ms=1e3 * t64, max_rel_residual=r64["max_rel_residual"],
-                           failed=r64["failed"]),
+                           failed=r64["failed"], sweeps_hist=np.bincount(r64["sweeps"]).tolist()),
+        "sweeps_hist": np.bincount(rmix["sweeps"]).tolist(),
         "speedup_vs_plain_fp64": t64 / tmix,
+        "plain_fp64_per_system": per_sys,
+        "speedup_vs_per_system_fp64": (per_sys["ms"] / (1e3 * tmix)) if "ms" in per_sys else None,
         "rel_residual_max": rmix["max_rel_residual"],
         "failed_systems": rmix["failed"],
         "refinement_sweeps_max": sweeps,
@@ -283,11 +342,13 @@ def main():
                     help="fp64: plain FP64 cyclic reduction; mixed: FP32 factors + FP64 refinement to 1e-12")
     ap.add_argument("--kappa", type=float, default=0.0,
                     help="if > 0, D_k = Q diag(s) Q^T with cond(D_k) log-uniform in [1e2, kappa]")
-    ap.add_argument("--workload", choices=["single", "batched", "selector"], default="single",
+    ap.add_argument("--workload", choices=["single", "batched", "selector"], default="batched",
                     help="single: one large system; batched: 4096 uneven systems; selector: 8192 systems with "
                          "the learned LU/QRP choice")
     ap.add_argument("--nsys", type=int, default=0)
     ap.add_argument("--cpu-sample-sys", type=int, default=48)
+    ap.add_argument("--per-system-sample", type=int, default=256,
+                    help="systems timed one at a time through the plain FP64 BorderedCR (batched workloads)")
     ap.add_argument("--pair-kernel", type=int, default=0,
                     help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n, "
                          "4 column-split waves for every n <= 64")
