@@ -71,6 +71,7 @@ def lib():
             "alglin_memcpy_dtoh": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtod": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtod_async": (ci, [vp, vp, i64, vp]),
+            "alglin_batch_time_factorize": (ci, [vp, vp, i32]),
             "alglin_batch_create": (vp, [ctypes.c_int]),
             "alglin_batch_destroy": (None, [vp]),
             "alglin_batch_plan": (ci, [vp, i32, vp]),
@@ -423,6 +424,7 @@ class Batch:
             raise AlglinError("alglin_batch_create failed")
         self.mixed = mixed
         self.ops = None
+        self._shared = False
 
     def __del__(self):
         try:
@@ -439,14 +441,20 @@ class Batch:
     def last_error(self) -> str:
         return lib().alglin_batch_last_error(self._h).decode()
 
-    def plan(self, dims):
+    def plan(self, dims, operator: bool = True):
+        """Lays the systems out; with operator=False no operator storage is
+        allocated (share_operator must follow before any solve)."""
         d = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 7))
         self.dims = d
         self._check(lib().alglin_batch_plan(self._h, len(d), _p(d)), "plan")
         # operator storage (device, packed)
-        if self.ops is not None:
+        if self.ops is not None and not getattr(self, "_shared", False):
             for a in self.ops.values():
                 a.free()
+        self.ops = None
+        self._shared = False
+        if not operator:
+            return
         sz = {k: max(1, int(lib().alglin_batch_size(self._h, i))) for i, k in enumerate(self.KINDS)}
         self.ops = dict(D=DeviceArray(sz["DE"]), E=DeviceArray(sz["DE"]), B=DeviceArray(sz["B"]),
                         C=DeviceArray(sz["C"]), H=DeviceArray(sz["H"]), Cq=DeviceArray(sz["Cq"]),
@@ -458,10 +466,11 @@ class Batch:
         """Use another batch's device operator (same plan) instead of our own copy."""
         if not np.array_equal(other.dims[:, :6], self.dims[:, :6]):
             raise AlglinError("share_operator: the two batches have different shapes")
-        for a in self.ops.values():
-            a.free()
-        self.ops = other.ops
-        self._shared_from = other
+        if self.ops is not None and not self._shared:
+            for a in self.ops.values():
+                a.free()
+        self.ops = other.ops      # the arrays stay alive as long as either batch holds them
+        self._shared = True
         ptr = [ctypes.c_void_p(self.ops[k].data_ptr()) for k in ("D", "E", "B", "C", "H", "Cq", "F")]
         self._check(lib().alglin_batch_bind_device(self._h, *ptr), "bind_device")
 
@@ -501,7 +510,7 @@ class Batch:
         q = np.zeros(ns, np.int32)
         b = np.zeros(ns, np.int32)
         self._check(lib().alglin_batch_selection(self._h, _p(f), _p(q), _p(b)), "selection")
-        return dict(features=f, chose_qrp=q.astype(bool), fell_back=b.astype(bool))
+        return dict(features=f, chose_qrp=q.astype(bool), fell_back=b.astype(bool), fallback_stage=b.copy())
 
     def load_system(self, s: int, sysd: dict):
         """Upload one system's blocks (system dict) into its packed slices."""
@@ -549,6 +558,16 @@ class Batch:
 
     def stream(self) -> int:
         return lib().alglin_batch_stream(self._h)
+
+    def time_factorize(self) -> np.ndarray:
+        """One eager factorization with every pair-elimination launch timed
+        (HIP events): rows of (level, class, workgroups, ms, bytes, flops)."""
+        cap = 4096
+        rows = np.zeros((cap, 6))
+        n = lib().alglin_batch_time_factorize(self._h, _p(rows), cap)
+        if n < 0:
+            raise AlglinError(f"time_factorize: {self.last_error()}")
+        return rows[: min(n, cap)].copy()
 
 
 class Mlp:

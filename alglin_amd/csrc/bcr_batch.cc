@@ -244,8 +244,18 @@ void BatchEngine<TA, TW>::build_factor_tables() {
       if (members.empty()) continue;
       int ppb = 1;
       if (cls <= 2) ppb = pair_lu_fast_ppb(kFastNmax[cls], nxmax, nrmax);
+      double lbytes = 0, lflops = 0;
       for (int s : members) {
         SysDims const& d = dims_[s];
+        {
+          // per pair: reads E_jp, D_jp, D_j, E_j, B_jp, B_j, C_j, C_jp; writes T, D, E (both rows),
+          // B (both), C_j, C_jp, pending C_jpp, the F partial and ipiv
+          const double n = d.n, nx = d.nx, nr = d.nr, np = (d.N + L - 1) / (2 * L);
+          const double rd = 4 * n * n + 2 * n * nx + 2 * nr * n;
+          const double wr = 6 * n * n + 2 * n * nx + 3 * nr * n + nr * nx;
+          lbytes += np * (rd * (L == 1 ? sizeof(TA) : sizeof(TW)) + wr * sizeof(TW) + 4 * n);
+          lflops += np * ((23.0 / 3.0) * n * n * n + 3 * n * n * nx + 5 * nr * n * n + 2 * nr * nx * n);
+        }
         FactorArgs<TW, TW> a{};
         a.N = d.N; a.n = d.n; a.nr = d.nr; a.nx = d.nx; a.L = L;
         a.npairs = (d.N + L - 1) / (2 * L);
@@ -282,6 +292,8 @@ void BatchEngine<TA, TW>::build_factor_tables() {
       p.l.nmax = cls <= 2 ? kFastNmax[cls] : nmax;
       p.l.nxmax = nxmax;
       p.l.nrmax = nrmax;
+      p.l.bytes = lbytes;
+      p.l.flops = lflops;
       pend[lev].push_back(p);
     }
   }
@@ -335,7 +347,7 @@ void BatchEngine<TA, TW>::build_factor_tables() {
 }
 
 template <typename TA, typename TW>
-void BatchEngine<TA, TW>::factorize_async() {
+void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
   if (!tables_ok_) build_factor_tables();
   const int ns = int(active_.size());
   if (ns == 0) return;
@@ -343,8 +355,10 @@ void BatchEngine<TA, TW>::factorize_async() {
   hip_check(launch_convert<int32_t, int32_t>(nullptr, skip_.p, nsys(), st_), "skip flags");
   hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
   hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
+  size_t li = 0;
   for (int k = 0; k < levels_; ++k)
     for (Launch const& l : fac_[k]) {
+      if (ev) hip_check(hipEventRecord((*ev)[2 * li], st_), "event");
       hipError_t e = hipSuccess;
       auto run = [&](auto const* args) {
         using A = std::remove_cv_t<std::remove_pointer_t<decltype(args)>>;
@@ -365,11 +379,42 @@ void BatchEngine<TA, TW>::factorize_async() {
       if (k == 0) run(static_cast<const FactorArgs<TA, TW>*>(l.args));
       else run(static_cast<const FactorArgs<TW, TW>*>(l.args));
       hip_check(e, "batch pair elimination");
+      if (ev) hip_check(hipEventRecord((*ev)[2 * li + 1], st_), "event");
+      ++li;
     }
   hip_check(launch_convert<TA, double>(pF_, Feff_.p, tot_[kbF], st_), "F");
   hip_check(launch_reduce_batched<TW, double>(fred_, ns, st_), "reduce F");
   hip_check(launch_assemble_last_batched<TW, TA>(asm_, ns, st_), "assemble last");
   hip_check(launch_lu_last_batched(lu_, stptr_, ns, Ncmax_, lds_limit_, st_), "lu last");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::factorize_async() {
+  factorize_impl(nullptr);
+}
+
+template <typename TA, typename TW>
+std::vector<typename BatchEngine<TA, TW>::LaunchTime> BatchEngine<TA, TW>::time_factorize() {
+  if (!tables_ok_) build_factor_tables();
+  std::vector<LaunchTime> out;
+  for (int k = 0; k < levels_; ++k)
+    for (Launch const& l : fac_[k]) out.push_back(LaunchTime{k, l.cls, l.nblocks, 0.0, l.bytes, l.flops});
+  std::vector<hipEvent_t> ev(2 * out.size(), nullptr);
+  for (auto& e : ev) hip_check(hipEventCreate(&e), "event create");
+  try {
+    factorize_impl(&ev);
+    hip_check(hipStreamSynchronize(st_), "timed factorize");
+    for (size_t i = 0; i < out.size(); ++i) {
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]), "event time");
+      out[i].ms = ms;
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return out;
 }
 
 template <typename TA, typename TW>
@@ -668,6 +713,15 @@ void BatchSolver::drop_graphs() {
 
 hipStream_t BatchSolver::stream() const { return mixed_ ? e32_->stream() : e64_->stream(); }
 
+std::vector<BatchEngine<double, float>::LaunchTime> BatchSolver::time_factorize() {
+  drop_graphs();
+  if (mixed_) return e32_->time_factorize();
+  std::vector<BatchEngine<double, float>::LaunchTime> out;
+  for (auto const& t : e64_->time_factorize())
+    out.push_back(BatchEngine<double, float>::LaunchTime{t.level, t.cls, t.nblocks, t.ms, t.bytes, t.flops});
+  return out;
+}
+
 void BatchSolver::plan(std::vector<SysDims> const& dims) {
   drop_graphs();
   if (mixed_) {
@@ -740,7 +794,7 @@ hipGraphExec_t capture(hipStream_t st, F&& body) {
 // of open systems shrinks; one int (the open count) comes back per sweep.
 template <typename E>
 void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done,
-                         BatchReport& rep) {
+                         BatchReport& rep, int giveup) {
   const int ns = eng.nsys();
   hipStream_t st = eng.stream();
   const bool graph = g_.res && g_.corr;
@@ -752,7 +806,7 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
   hip_check(hipMemcpyAsync(drel_.p, rel_.data(), d8, hipMemcpyHostToDevice, st), "residuals");
   hip_check(hipMemcpyAsync(dsweeps_.p, sweeps_.data(), i4, hipMemcpyHostToDevice, st), "sweeps");
   ConvergeArgs ca{norms_.p, dstat_.p, dbest_.p, dstag_.p, eng.skip_flags(), drel_.p, dsweeps_.p, dopen_.p, tol, 0,
-                  ns};
+                  ns, giveup};
   int32_t open = 0;
   for (int it = 0;; ++it) {
     if (graph) {
@@ -894,42 +948,56 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       eng.set_active(std::vector<char>(ns, 1));
     }
   }
-  // 3. FP64 refinement of every system with its factors
+  // 3. FP64 refinement of every system with its factors; with the fallback
+  // behind it, a system that would need more than six further sweeps at its
+  // contraction rate is handed over at once instead of sweeping to max_iter
   std::vector<char> done(ns, 0);
-  refine(eng, B, X, tol, max_iter, done, rep);
-  // 4. FP64 QRP for the systems that did not get there
-  std::vector<char> bad(ns, 0);
-  int nbad = 0;
-  for (int s = 0; s < ns; ++s)
-    if (status_[s] != 0 || !(rel_[s] <= tol)) { bad[s] = 1; ++nbad; }
-  if (fallback && nbad > 0) {
-    // FP64 QRP elimination, refined in FP64 (its factors lose digits on
-    // nearly singular pair columns; the refinement recovers them while the
-    // FP64 backward error stays below one)
+  refine(eng, B, X, tol, max_iter, done, rep, fallback ? 6 : 0);
+  // 4. the FP64 fallback for the systems that did not get there: first the
+  // plain FP64 path (LU elimination, FP64 refinement), then FP64 QRP for what
+  // FP64 LU cannot solve either (QRP's factors lose digits on nearly singular
+  // pair columns; the refinement recovers them while the FP64 backward error
+  // stays below one).  fell_back_ = 1 (FP64 LU) or 2 (FP64 QRP).
+  auto failed = [&](std::vector<char> const& among, std::vector<char>& out) {
+    int k = 0;
+    for (int s = 0; s < ns; ++s) {
+      out[s] = among[s] && (status_[s] != 0 || !(rel_[s] <= tol));
+      k += out[s];
+    }
+    return k;
+  };
+  std::vector<char> all(ns, 1), bad(ns, 0);
+  if (fallback && failed(all, bad) > 0) {
     if (!e64_) {
       e64_ = new BatchEngine<double, double>();
       e64_->plan(dims_);
       e64_->bind_device(oD_, oE_, oB_, oC_, oH_, oCq_, oF_);
     }
     auto& f = *e64_;
-    for (int s = 0; s < ns; ++s) f.set_method(s, kQRP);
-    f.set_active(bad);
     hip_check(hipStreamSynchronize(st), "sync before fallback");
-    f.copy_active_async(B, X);
-    f.factorize_async();
-    f.solve_async(X);
-    std::vector<int32_t> st3(ns, 0);
-    f.status(st3.data());
-    std::vector<char> done2(ns, 1);
-    for (int s = 0; s < ns; ++s)
-      if (bad[s]) {
-        fell_back_[s] = 1;
-        status_[s] = st3[s];
-        done2[s] = 0;
-      }
-    BatchReport r2;
-    refine(f, B, X, tol, max_iter, done2, r2);
-    rep.sweeps = std::max(rep.sweeps, r2.sweeps);
+    for (int stage = 1; stage <= 2; ++stage) {
+      for (int s = 0; s < ns; ++s) f.set_method(s, stage == 1 ? kLU : kQRP);
+      f.set_active(bad);
+      f.copy_active_async(B, X);
+      f.factorize_async();
+      f.solve_async(X);
+      std::vector<int32_t> st3(ns, 0);
+      f.status(st3.data());
+      std::vector<char> done2(ns, 1);
+      for (int s = 0; s < ns; ++s)
+        if (bad[s]) {
+          fell_back_[s] = char(stage);
+          status_[s] = st3[s];
+          done2[s] = 0;
+        }
+      BatchReport r2;
+      refine(f, B, X, tol, max_iter, done2, r2);
+      rep.sweeps = std::max(rep.sweeps, r2.sweeps);
+      std::vector<char> still(ns, 0);
+      if (stage == 1 && failed(bad, still) == 0) break;
+      bad.swap(still);
+    }
+    f.set_active(all);
   }
   for (int s = 0; s < ns; ++s) {
     rep.max_rel_residual = std::max(rep.max_rel_residual, rel_[s]);

@@ -97,12 +97,23 @@ class BatchEngine {
   // launches skip a system whose flag is nonzero; factorize_async clears them
   int32_t* skip_flags() { return skip_.p; }
 
+  // one eager factorization with HIP events around every pair-elimination
+  // launch: per launch its level, kernel class, workgroups, duration and
+  // algorithmic bytes / flops (reads of the level-0 sources in TA, the rest
+  // in TW)
+  struct LaunchTime {
+    int level, cls, nblocks;
+    double ms, bytes, flops;
+  };
+  std::vector<LaunchTime> time_factorize();
+
  private:
   struct Launch {
     const void* args;
     const int32_t* entry;
     const int32_t* first;
     int nblocks, cls, method, nmax, nxmax, nrmax;
+    double bytes = 0, flops = 0;   // algorithmic HBM bytes and flops of a factor launch
   };
   struct SolveTables {
     std::vector<std::vector<Launch>> fwd;   // per level
@@ -127,6 +138,7 @@ class BatchEngine {
     DevBuf<char> blob;
   };
 
+  void factorize_impl(std::vector<hipEvent_t>* ev);
   void build_factor_tables();
   const CopyArgs* add_table(double const* src, double* dst);
   SolveTables& solve_tables(double* X);
@@ -190,7 +202,8 @@ class BatchSolver {
   // mixed solver with per-system method choice: every system starts with FP32
   // LU; after the first solve the selector (if given) sends the systems it
   // scores above `threshold` to FP32 QRP; all are refined in FP64; with
-  // `fallback`, systems that still miss tol are solved by FP64 QRP
+  // `fallback`, systems that still miss tol are solved in FP64: LU (the plain
+  // FP64 path) first, QRP for what that misses
   BatchReport solve_selected(double const* B, double* X, double tol, int max_iter, class Mlp* selector,
                              float threshold, bool fallback);
   // features of the last solve_selected (12 per system, host) and the
@@ -205,6 +218,9 @@ class BatchSolver {
   std::vector<int32_t> const& sweeps() const { return sweeps_; }
   hipStream_t stream() const;
   bool mixed() const { return mixed_; }
+  // one eager factorization of the solver's engine (FP32 factors when
+  // mixed) with every pair-elimination launch timed by HIP events
+  std::vector<BatchEngine<double, float>::LaunchTime> time_factorize();
 
  private:
   template <typename E> BatchReport run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph);
@@ -226,7 +242,8 @@ class BatchSolver {
   // refinement loop over the systems of `eng` not marked done (device
   // convergence test; done systems are skipped by every launch)
   template <typename E>
-  void refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done, BatchReport& rep);
+  void refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done, BatchReport& rep,
+              int giveup = 0);
   // captured graphs: factor+solve, residual+norms, correction
   struct Graphs {
     const double* B = nullptr;

@@ -70,7 +70,9 @@ hipError_t launch_add_batched(const CopyArgs* args, int nsys, hipStream_t st);
 // norms[2s], norms[2s+1]; a system not yet done becomes done when it has
 // converged (rel <= tol), is not finite, its factorization failed
 // (status != 0) or it stalled (three sweeps in a row without halving its best
-// residual); *open counts the systems left open (cleared by the caller).
+// residual); with giveup > 0 also when, at its last contraction rate, it
+// would need more than giveup further sweeps (a better solver takes over);
+// *open counts the systems left open (cleared by the caller).
 struct ConvergeArgs {
   const double* norms;
   const int32_t* status;
@@ -83,6 +85,7 @@ struct ConvergeArgs {
   double tol;
   int it;      // sweep index; 0 resets best / stagnant
   int nsys;
+  int giveup;  // 0: off
 };
 hipError_t launch_converge_batched(ConvergeArgs const& a, hipStream_t st);
 

@@ -207,6 +207,7 @@ __global__ void __launch_bounds__(256) converge_batched(ConvergeArgs a) {
   if (s >= a.nsys || a.done[s]) return;
   const double nb = a.norms[2 * s] > 0 ? a.norms[2 * s] : 1.0;
   const double rel = a.norms[2 * s + 1] / nb;
+  const double prev = a.rel[s];
   a.rel[s] = rel;
   a.sweeps[s] = a.it;
   if (a.status[s] != 0 || !(rel == rel) || rel <= a.tol) {
@@ -226,6 +227,14 @@ __global__ void __launch_bounds__(256) converge_batched(ConvergeArgs a) {
   }
   a.best[s] = fmin(best, rel);
   a.stagnant[s] = stag;
+  if (a.giveup > 0 && a.it >= 1 && prev > 0 && rel < prev) {
+    // sweeps still needed at the last contraction rate
+    const double need = log(a.tol / rel) / log(rel / prev);
+    if (need > double(a.giveup)) {
+      a.done[s] = 1;
+      return;
+    }
+  }
   atomicAdd(a.open, 1);
 }
 

@@ -669,6 +669,25 @@ int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual,
 
 char const* alglin_batch_last_error(alglin_batch_t h) { return h ? h->err.c_str() : "null handle"; }
 
+int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows) {
+  int n = -1;
+  const int rc = bguard(h, [&] {
+    auto t = h->s->time_factorize();
+    n = int(t.size());
+    for (int i = 0; i < n && i < max_rows; ++i) {
+      double* r = rows + 6 * size_t(i);
+      r[0] = t[i].level;
+      r[1] = t[i].cls;
+      r[2] = t[i].nblocks;
+      r[3] = t[i].ms;
+      r[4] = t[i].bytes;
+      r[5] = t[i].flops;
+    }
+    return 0;
+  });
+  return rc == 0 ? n : -1;
+}
+
 void* alglin_batch_stream(alglin_batch_t h) { return h ? static_cast<void*>(h->s->stream()) : nullptr; }
 
 int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods) {

@@ -14,7 +14,7 @@ baseline (the oracle on a sample of the systems, one system per host thread).
 
 Other workloads:
   --workload selector  8192 systems, half nearly singular, the learned LU/QRP
-                       selector with the FP64 QRP fallback, against the mixed
+                       selector with the FP64 fallback (LU, then QRP), against the mixed
                        solve without the selector and against plain FP64;
   --workload single    one large system (N = 100000, n = 32, qr = qx = 8,
                        nr = nx = 4, shift 3, b = A * ones) in plain FP64 or
@@ -207,7 +207,7 @@ def time_plain_per_system(bt, dims, B, sample, seed):
 def bench_batched(args):
     """Batched uneven systems: plain FP64 cyclic reduction vs FP32 factors +
     FP64 refinement (graph replay), or, for the selector workload, vs the
-    mixed solve with the learned LU/QRP choice and the FP64 QRP fallback."""
+    mixed solve with the learned LU/QRP choice and the FP64 fallback (LU, then QRP)."""
     from alglin_amd import selector as S
     from alglin_amd import systems
     from alglin_amd.bcr import Batch, DeviceArray, device_info
@@ -225,7 +225,7 @@ def bench_batched(args):
     ref.plan(dims)
     ref.generate(seed=args.seed, shift=4.0, sing_k=sing, eps=eps)
     mix = Batch(mixed=True)
-    mix.plan(dims)
+    mix.plan(dims, operator=False)
     mix.share_operator(ref)
     L = ref.size("x")
     B = DeviceArray.from_host(np.random.default_rng(args.seed + 5).standard_normal(L))
@@ -245,6 +245,11 @@ def bench_batched(args):
     tol = 1e-12
     t64 = timed(lambda: ref.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
     r64 = ref.report()
+    try:
+        per_sys = time_plain_per_system(mix, dims, B, args.per_system_sample, args.seed)
+    except Exception as e:   # reported, not fatal
+        per_sys = {"error": str(e)}
+    del ref   # its FP64 factors; the operator stays alive through mix
     out = {}
     if sel:
         mlp = S.load()
@@ -260,19 +265,33 @@ def bench_batched(args):
                                              max_rel_residual=r_none["max_rel_residual"])
         out["selector"] = dict(sent_to_qrp=int(chosen["chose_qrp"].sum()),
                                fp64_fallbacks=int(chosen["fell_back"].sum()),
+                               fp64_lu=int((chosen["fallback_stage"] == 1).sum()),
+                               fp64_qrp=int((chosen["fallback_stage"] == 2).sum()),
                                singular_systems=int((sing >= 1).sum()))
     else:
         tmix = timed(lambda: mix.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
         rmix = mix.report()
-    try:
-        per_sys = time_plain_per_system(ref, dims, B, args.per_system_sample, args.seed)
-    except Exception as e:   # reported, not fatal
-        per_sys = {"error": str(e)}
     sweeps = int(rmix["max_sweeps"])
     fac = batch_pair_bytes(dims, 4)
     sw, res = batch_sweep_bytes(dims, 4)
     nbytes = fac + (1 + sweeps) * sw + (1 + sweeps) * res
-    achieved = nbytes / tmix / 1e9
+    # dominant kernel: the pair-elimination launch class with the most time in
+    # one eager factorization, every launch timed with HIP events on the
+    # batch's stream
+    tf = mix.time_factorize()
+    cls_names = {0: "pair_lu_fast_batched (n <= 4)", 1: "pair_lu_fast_batched (n <= 8)",
+                 2: "pair_lu_fast_batched (n <= 16)", 3: "pair_lu_split_batched (16 < n <= 32)",
+                 4: "pair_lu_cols_batched (32 < n <= 64)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
+    per_cls = {}
+    for lev, cls, nb, ms, by, fl in tf:
+        c = per_cls.setdefault(int(cls), [0, 0.0, 0.0, 0.0])
+        c[0] += 1
+        c[1] += ms
+        c[2] += by
+        c[3] += fl
+    dom = max(per_cls, key=lambda k: per_cls[k][1])
+    nl, ms_d, by_d, fl_d = per_cls[dom]
+    achieved = (by_d / nl) / (ms_d / nl * 1e-3) / 1e9
     print(f"[bench] plain FP64 {1e3 * t64:.1f} ms, mixed{'+learned' if sel else ''} {1e3 * tmix:.1f} ms, "
           f"max residual {rmix['max_rel_residual']:.2e}", file=sys.stderr, flush=True)
     result = {
@@ -290,7 +309,7 @@ def bench_batched(args):
         "data": "synthetic (counter-based Gaussian blocks generated on the GPU, random b)",
         "config": {
             "workload": (f"{nsys} uneven BABD systems, n in {{8,16,32}}, N in [64,2048], half with a rank-deficient "
-                         f"pair column (1e-10), mixed + learned LU/QRP selector + FP64 QRP fallback" if sel else
+                         f"pair column (1e-10), mixed + learned LU/QRP selector + FP64 fallback (LU, then QRP)" if sel else
                          f"{nsys} uneven BABD systems, n in 4..48, N log-uniform [16,4096], qr=qx=n/4, nr=nx=2, "
                          f"shift 4, FP32 factors + FP64 refinement, graph replay"),
             "nsys": nsys, "tol": tol, "parallelism": "single-gpu",
@@ -306,20 +325,25 @@ def bench_batched(args):
         "refinement_sweeps_max": sweeps,
         "roofline": {
             "bound": "hbm",
-            "kernel": "whole batched solve (pair eliminations + sweeps + FP64 residuals), algorithmic bytes / wall time",
+            "kernel": f"{cls_names.get(dom, dom)}, FP32 factors, one launch per level",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_step": nbytes,
+            "bytes_per_launch": by_d / nl,
+            "avg_launch_ms": ms_d / nl,
+            "launches": nl,
+            "flop_rate_tflops": fl_d / (ms_d * 1e-3) / 1e12,
+            "factorize_ms_by_class": {cls_names.get(k, str(k)): v[1] for k, v in sorted(per_cls.items())},
+            "whole_solve_algorithmic_gbs": nbytes / tmix / 1e9,
         },
         "device": device_info(),
     }
     result.update(out)
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = run_cpu_baseline_batch(ref, dims, min(nsys, args.cpu_sample_sys))
+            result["cpu_baseline"] = run_cpu_baseline_batch(mix, dims, min(nsys, args.cpu_sample_sys))
         except Exception as e:  # the baseline is reported, never required
             result["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(result))

@@ -195,6 +195,13 @@ int alglin_batch_solve(alglin_batch_t h, double const* B, double* X, double tol,
 int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual, int32_t* sweeps, double summary[4]);
 char const* alglin_batch_last_error(alglin_batch_t h);
 void* alglin_batch_stream(alglin_batch_t h);
+/* One eager factorization (FP32 factors for a mixed batch) with HIP events
+   around every pair-elimination launch.  Row i of `rows` (6 doubles, at most
+   max_rows rows) = level, kernel class, workgroups, milliseconds,
+   algorithmic HBM bytes, flops of launch i.  Returns the number of launches
+   (may exceed max_rows), -1 on error.  No counterpart in the reference: the
+   measurement hook of the batched extension. */
+int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows);
 
 /* per-system pair elimination method for alglin_batch_solve (0 LU, 1 QR, 2 QRP) */
 int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods);
@@ -227,11 +234,13 @@ int alglin_mlp_predict(alglin_mlp_t h, float const* X, int32_t n, float* p1);
 /* mixed batch solve with per-system method choice: FP32 LU first; systems the
    selector scores above threshold (selector may be null: none) are
    refactored with FP32 QRP; FP64 refinement; with fallback=1 the systems that
-   still miss tol are solved by FP64 QRP.  Return codes as alglin_batch_solve. */
+   still miss tol are solved in FP64: the plain FP64 path (LU) first, FP64
+   QRP for what that misses.  Return codes as alglin_batch_solve. */
 int alglin_batch_solve_selected(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter,
                                 alglin_mlp_t selector, float threshold, int fallback);
 /* of the last solve_selected: features (12 per system), systems sent to QRP,
-   systems solved by the FP64 fallback (host arrays; any may be null) */
+   FP64 fallback stage per system (0 none, 1 FP64 LU, 2 FP64 QRP) (host
+   arrays; any may be null) */
 int alglin_batch_selection(alglin_batch_t h, float* features, int32_t* chose_qrp, int32_t* fell_back);
 
 #ifdef __cplusplus

@@ -6,8 +6,8 @@
 * GPU training lowers the loss and separates a learnable rule;
 * solver-generated samples: the selector trained on them and applied to a
   held-out batch never does worse than the method without it (LU first,
-  FP64 QRP fallback for the systems that miss tol) and every system ends at
-  the tolerance."""
+  the FP64 fallback -- LU, then QRP -- for the systems that miss tol) and
+  every system ends at the tolerance."""
 import numpy as np
 import pytest
 
