@@ -111,14 +111,19 @@ __device__ __forceinline__ void update_columns(TW (&vt)[RS][KT], TW (&vo)[RS][KO
       for (int s = 0; s < RS; ++s) vt[s][k] = upd[s] ? lv[s] : vt[s][k];
     }
   }
+  // right-hand columns in groups of four behind one guard: a padding column
+  // (beyond 2n + nx) holds zeros and stays zero, so a group is updated whole
 #pragma unroll
-  for (int k = 0; k < KO; ++k) {
-    if (w + NW * k < no) {
-      const TW u = wave::rl(sp ? vo[S][k] : vo[0][k], who);
+  for (int k0 = 0; k0 < KO; k0 += 4) {
+    if (w + NW * k0 < no) {
 #pragma unroll
-      for (int s = 0; s < RS; ++s) vo[s][k] = fma(-lv[s], u, vo[s][k]);
+      for (int k = k0; k < k0 + 4 && k < KO; ++k) {
+        const TW u = wave::rl(sp ? vo[S][k] : vo[0][k], who);
+#pragma unroll
+        for (int s = 0; s < RS; ++s) vo[s][k] = fma(-lv[s], u, vo[s][k]);
+      }
     }
-    if ((k & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -198,7 +203,12 @@ __device__ __forceinline__ void pair_lu_cols_body(FactorArgs<TS, TW> const& a, i
       lv[s] = upd[s] ? col[s] * rinv : TW(0);
     }
     TW* next = colbuf[(c + 1) & 1];
-    update_columns<NW>(vt, vo, lv, upd, sp != 0, who, w, c, n, no, next, lane);
+    // the column guards below compare compile-time k with these; opaque
+    // copies keep the compiler from hoisting every guard out of the step loop
+    // into its own SGPRs (which then spill and are reloaded each step)
+    int w_ = w, n_ = n, no_ = no;
+    asm volatile("" : "+s"(w_), "+s"(n_), "+s"(no_));
+    update_columns<NW>(vt, vo, lv, upd, sp != 0, who, w_, c, n_, no_, next, lane);
     // column c + 1 published; every wave is done reading column c's slot
     __syncthreads();
   }
@@ -236,10 +246,12 @@ __global__ void __launch_bounds__(64 * NW) pair_lu_cols_batched(BatchTable<Facto
   pair_lu_cols_body<TS, TW, NMAX, NXMAX, NW, RS>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
-// waves per workgroup: FP64 values take two VGPRs, so twice the waves keep
-// about 80 panel registers per lane in both precisions
+// waves per workgroup: eight for n <= 48 (about 40 panel registers per lane
+// in FP32, 80 in FP64: 7 and 4 waves per SIMD), 8 / 16 for n <= 64 (about 80
+// registers); more waves per pair shorten each step's column loop, which is
+// the step's critical path
 template <typename TW> struct ColsWaves;
-template <> struct ColsWaves<float> { static constexpr int k48 = 4, k64 = 8; };
+template <> struct ColsWaves<float> { static constexpr int k48 = 8, k64 = 8; };
 template <> struct ColsWaves<double> { static constexpr int k48 = 8, k64 = 16; };
 
 // 1: n <= 48, nx <= 4, 2n + nr <= 128 (two row sets); 2: n <= 64, nx <= 16,
