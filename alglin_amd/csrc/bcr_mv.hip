@@ -9,26 +9,61 @@
 // The per-block partials make the border rows a deterministic reduction
 // instead of N serial gemv accumulations.
 #include "bcr_launch.hh"
+#include "bcr_wave.hh"
 
 namespace bcr {
 
 
+// Loads are issued eight columns at a time from clamped addresses and masked
+// after (keep_if), so each chunk costs one memory latency instead of one per
+// column; the sums run in the original column order.
 template <typename TA>
 __device__ __forceinline__ void mv_rows_body(MvArgs<TA> a, int bid) {
+  constexpr int CH = 8;
   const int lane = threadIdx.x & 63;
-  const int k = bid * 4 + (threadIdx.x >> 6);
+  // wave-uniform block row (made explicit: the row's addresses stay scalar)
+  const int k = __builtin_amdgcn_readfirstlane(bid * 4 + int(threadIdx.x >> 6));
   const int N = a.N, n = a.n, nx = a.nx, nr = a.nr;
   if (k > N) return;
   const double* xk = a.x + size_t(k) * n;
   const double* xb = a.x + size_t(N + 1) * n + a.qx;
+  // x_k (and x_{k+1} below) one entry per lane (n <= 64), broadcast per
+  // column by v_readlane instead of one vector load per column
+  const double xa = lane < n ? xk[lane] : 0.0;
   if (k < N) {
     const TA* Dk = a.D + size_t(k) * n * n;
     const TA* Ek = a.E + size_t(k) * n * n;
     const TA* Bk = a.B + size_t(k) * n * nx;
+    const double xn = lane < n ? xk[n + lane] : 0.0;
     for (int r = lane; r < n; r += 64) {
       double acc = 0;
-      for (int c = 0; c < n; ++c) acc += double(Dk[r + c * n]) * xk[c] + double(Ek[r + c * n]) * xk[n + c];
-      for (int c = 0; c < nx; ++c) acc += double(Bk[r + c * n]) * xb[c];
+      for (int c0 = 0; c0 < n; c0 += CH) {
+        TA d[CH], e[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = c0 + i < n ? c0 + i : 0;
+          d[i] = Dk[r + cc * n];
+          e[i] = Ek[r + cc * n];
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = c0 + i < n ? c0 + i : 0;
+          const double u = wave::rl(xa, cc), v = wave::rl(xn, cc);
+          acc += keep_if(double(d[i]) * u + double(e[i]) * v, c0 + i < n);
+        }
+      }
+      for (int c0 = 0; c0 < nx; c0 += CH) {
+        TA bb[CH];
+        double w[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = c0 + i < nx ? c0 + i : 0;
+          bb[i] = Bk[r + cc * n];
+          w[i] = xb[cc];
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) acc += keep_if(double(bb[i]) * w[i], c0 + i < nx);
+      }
       const size_t row = size_t(k) * n + r;
       a.out[row] = a.b ? a.b[row] - acc : acc;
     }
@@ -36,7 +71,18 @@ __device__ __forceinline__ void mv_rows_body(MvArgs<TA> a, int bid) {
   const TA* Ck = a.C + size_t(k) * nr * n;
   for (int i = lane; i < nr; i += 64) {
     double acc = 0;
-    for (int c = 0; c < n; ++c) acc += double(Ck[i + c * nr]) * xk[c];
+    for (int c0 = 0; c0 < n; c0 += CH) {
+      TA cv[CH];
+      double u[CH];
+#pragma unroll
+      for (int t = 0; t < CH; ++t) {
+        const int cc = c0 + t < n ? c0 + t : 0;
+        cv[t] = Ck[i + cc * nr];
+        u[t] = wave::rl(xa, cc);
+      }
+#pragma unroll
+      for (int t = 0; t < CH; ++t) acc += keep_if(double(cv[t]) * u[t], c0 + t < n);
+    }
     a.Cpart[size_t(k) * nr + i] = acc;
   }
 }

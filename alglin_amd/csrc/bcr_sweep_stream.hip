@@ -56,23 +56,30 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
   auto in_row = [&](int r) { return r < n ? size_t(jp) * n + r : size_t(j) * n + (r - n); };
   auto out_row = [&](int r) { return r < n ? size_t(j) * n + r : size_t(jp) * n + (r - n); };
   const TW* Cj = a.C + size_t(j) * nr * n;
+  // factor columns eight at a time, from clamped addresses: one memory
+  // latency per chunk (rows past 2n hold garbage that is never used)
+  const int r0c = v0 ? r0 : 0, r1c = v1 ? r1 : 0;
   for (int q = 0; q < nrhs; ++q) {
     TX* x = a.x + size_t(q) * a.ldx;
     TX x0 = v0 ? x[in_row(src0)] : TX(0);
     TX x1 = v1 ? x[in_row(src1)] : TX(0);
-    TW t0 = v0 ? Tj[r0] : TW(0), t1 = v1 ? Tj[r1] : TW(0);
-    for (int c = 0; c < n; ++c) {
-      // next column's factors in flight while this one is applied
-      TW n0 = TW(0), n1 = TW(0);
-      if (c + 1 < n) {
-        n0 = v0 ? Tj[r0 + size_t(c + 1) * n2] : TW(0);
-        n1 = v1 ? Tj[r1 + size_t(c + 1) * n2] : TW(0);
+    for (int c0 = 0; c0 < n; c0 += 8) {
+      TW t0[8], t1[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const size_t cc = size_t(c0 + i < n ? c0 + i : 0) * n2;
+        t0[i] = Tj[r0c + cc];
+        t1[i] = Tj[r1c + cc];
       }
-      const TX xc = TX(rlane(double(x0), c));
-      if (r0 > c) x0 -= TX(t0) * xc;
-      x1 -= TX(t1) * xc;
-      t0 = n0;
-      t1 = n1;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = c0 + i;
+        if (c < n) {
+          const TX xc = TX(rlane(double(x0), c));
+          if (r0 > c) x0 -= TX(t0[i]) * xc;
+          x1 -= TX(t1[i]) * xc;
+        }
+      }
     }
     if (v0) x[out_row(r0)] = x0;
     if (v1) x[out_row(r1)] = x1;
@@ -98,33 +105,47 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
   const bool row = lane < n;
   const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
+  const int rc = row ? lane : 0;   // lanes past n read row 0 and never store
   for (int q = 0; q < nrhs; ++q) {
     TX* x = a.x + size_t(q) * a.ldx;
     const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
     const TX vjpp = row ? x[size_t(jpp) * n + lane] : TX(0);
     TX s = row ? x[size_t(j) * n + lane] : TX(0);
-    TW d = row ? Dj[lane] : TW(0), e = row ? Ej[lane] : TW(0);
-    for (int c = 0; c < n; ++c) {
-      TW dn = TW(0), en = TW(0);
-      if (c + 1 < n && row) {
-        dn = Dj[lane + size_t(c + 1) * n];
-        en = Ej[lane + size_t(c + 1) * n];
+    // D_j, E_j columns eight at a time (one memory latency per chunk)
+    for (int c0 = 0; c0 < n; c0 += 8) {
+      TW d[8], e[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const size_t cc = size_t(c0 + i < n ? c0 + i : 0) * n;
+        d[i] = Dj[rc + cc];
+        e[i] = Ej[rc + cc];
       }
-      s -= TX(d) * TX(rlane(double(vjp), c)) + TX(e) * TX(rlane(double(vjpp), c));
-      d = dn;
-      e = en;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = c0 + i;
+        if (c < n) s -= TX(d[i]) * TX(rlane(double(vjp), c)) + TX(e[i]) * TX(rlane(double(vjpp), c));
+      }
     }
     for (int c = 0; c < nx; ++c) s -= TX(row ? Bj[lane + size_t(c) * n] : TW(0)) * x[xbrow + c];
-    // U^{-1}, bottom up: lane c owns the diagonal of column c
-    TW u = row ? Tj[lane + size_t(n - 1) * n2] : TW(0);
-    for (int c = n - 1; c >= 0; --c) {
-      TW un = TW(0);
-      if (c > 0 && row) un = Tj[lane + size_t(c - 1) * n2];
-      const TX den = lane == c ? TX(u) : TX(1);
-      const TX xc = TX(rlane(double(s / den), c));
-      if (lane < c) s -= TX(u) * xc;
-      else if (lane == c) s = xc;
-      u = un;
+    // U^{-1}, bottom up: lane c owns the diagonal of column c; columns eight
+    // at a time from the top of the remaining range down
+    for (int c1 = n - 1; c1 >= 0; c1 -= 8) {
+      TW u[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = c1 - i;
+        u[i] = Tj[rc + size_t(c >= 0 ? c : 0) * n2];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int c = c1 - i;
+        if (c >= 0) {
+          const TX den = lane == c ? TX(u[i]) : TX(1);
+          const TX xc = TX(rlane(double(s / den), c));
+          if (lane < c) s -= TX(u[i]) * xc;
+          else if (lane == c) s = xc;
+        }
+      }
     }
     if (row) x[size_t(j) * n + lane] = s;
   }
