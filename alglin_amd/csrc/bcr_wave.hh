@@ -11,29 +11,32 @@ namespace wave {
 constexpr int kQuad1032 = 0xB1, kQuad2301 = 0x4E, kHalfMirror = 0x141, kMirror = 0x140;
 constexpr int kBcast15 = 0x142, kBcast31 = 0x143;
 
+// x from the DPP source lane; lanes the row mask leaves out get `id` (the
+// identity of the reduction), so the move can fold into the max / min
 template <int CTRL, int ROWMASK = 0xf>
-__device__ __forceinline__ unsigned mov(unsigned x) {
-  return unsigned(__builtin_amdgcn_update_dpp(int(x), int(x), CTRL, ROWMASK, 0xf, false));
+__device__ __forceinline__ unsigned mov(unsigned x, unsigned id) {
+  return unsigned(__builtin_amdgcn_update_dpp(int(id), int(x), CTRL, ROWMASK, 0xf, false));
 }
 
 // max over the 64 lanes, returned uniform
 __device__ __forceinline__ unsigned max_u32(unsigned x) {
-  x = max(x, mov<kQuad1032>(x));
-  x = max(x, mov<kQuad2301>(x));
-  x = max(x, mov<kHalfMirror>(x));
-  x = max(x, mov<kMirror>(x));
-  x = max(x, mov<kBcast15, 0xa>(x));
-  x = max(x, mov<kBcast31, 0xc>(x));
+  x = max(x, mov<kQuad1032>(x, 0u));
+  x = max(x, mov<kQuad2301>(x, 0u));
+  x = max(x, mov<kHalfMirror>(x, 0u));
+  x = max(x, mov<kMirror>(x, 0u));
+  x = max(x, mov<kBcast15, 0xa>(x, 0u));
+  x = max(x, mov<kBcast31, 0xc>(x, 0u));
   return unsigned(__builtin_amdgcn_readlane(int(x), 63));
 }
 
 __device__ __forceinline__ unsigned min_u32(unsigned x) {
-  x = min(x, mov<kQuad1032>(x));
-  x = min(x, mov<kQuad2301>(x));
-  x = min(x, mov<kHalfMirror>(x));
-  x = min(x, mov<kMirror>(x));
-  x = min(x, mov<kBcast15, 0xa>(x));
-  x = min(x, mov<kBcast31, 0xc>(x));
+  constexpr unsigned id = 0xffffffffu;
+  x = min(x, mov<kQuad1032>(x, id));
+  x = min(x, mov<kQuad2301>(x, id));
+  x = min(x, mov<kHalfMirror>(x, id));
+  x = min(x, mov<kMirror>(x, id));
+  x = min(x, mov<kBcast15, 0xa>(x, id));
+  x = min(x, mov<kBcast31, 0xc>(x, id));
   return unsigned(__builtin_amdgcn_readlane(int(x), 63));
 }
 
