@@ -5,17 +5,16 @@
 // Wave w holds the T columns w + NW k (k < KT) and the right-hand columns
 // n + w + NW k (k < KO).
 //
-// Per elimination step c there is one barrier:
-//  * every wave reads column c (published in LDS by its owner) for its lanes'
-//    rows and finds the pivot itself: a DPP max-reduction of |a_rc| over the
-//    rows at positions >= c among the first 2n, ties to the smaller position
-//    (LAPACK idamax order).  The row positions live in registers and are the
-//    same in every wave, so no wave waits for another to pick the pivot;
-//  * the pivot row's entry of each owned column is broadcast from the pivot
-//    lane by v_readlane (every wave holds every row of its columns), so the
-//    pivot row is never published;
-//  * the owner of column c + 1 publishes it, updated, in the other LDS slot
-//    (the step's one barrier follows every wave's updates).  Finished T
+// Per elimination step c, two barriers:
+//  * wave 0 reads column c (published in LDS by its owner), finds the pivot:
+//    a DPP max-reduction of |a_rc| over the rows at positions >= c among the
+//    first 2n, ties to the smaller position (LAPACK idamax order), keeps the
+//    row positions, and publishes the multipliers, the final column c and
+//    the pivot lane;
+//  * every wave then takes the pivot row's entry of each owned column from
+//    the pivot lane by v_readlane (every wave holds every row of its
+//    columns, so the pivot row is never published) and updates; the owner of
+//    column c + 1 publishes it, updated, in the other LDS slot.  Finished T
 //    columns (< c) are skipped.
 // Rows are not moved: each is stored at its final position at the end.  The
 // arithmetic is the generic kernel's (multiplier = a_rc * (1/pivot), fma
@@ -89,11 +88,12 @@ __device__ __forceinline__ void store_elem(FactorArgs<TS, TW> const& a, int q, i
   }
 }
 
-// the column updates of step c with the pivot row in row set SP, lane `who`
+// the column updates of step c with the pivot row in row set sp, lane `who`;
+// mc = the final values of column c (multipliers below the pivot)
 template <int NW, int RS, int KT, int KO, typename TW>
 __device__ __forceinline__ void update_columns(TW (&vt)[RS][KT], TW (&vo)[RS][KO], const TW (&lv)[RS],
-                                               const bool (&upd)[RS], bool sp, int who, int w, int c, int n,
-                                               int no, TW* next, int lane) {
+                                               const TW (&mc)[RS], bool sp, int who, int w, int c, int n, int no,
+                                               TW* next, int lane) {
   constexpr int S = RS > 1 ? 1 : 0;
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
@@ -108,7 +108,7 @@ __device__ __forceinline__ void update_columns(TW (&vt)[RS][KT], TW (&vo)[RS][KO
       }
     } else if (col == c) {   // column c becomes the multipliers
 #pragma unroll
-      for (int s = 0; s < RS; ++s) vt[s][k] = upd[s] ? lv[s] : vt[s][k];
+      for (int s = 0; s < RS; ++s) vt[s][k] = mc[s];
     }
   }
   // right-hand columns in groups of four behind one guard: a padding column
@@ -132,6 +132,9 @@ __device__ __forceinline__ void pair_lu_cols_body(FactorArgs<TS, TW> const& a, i
   constexpr int KT = (NMAX + NW - 1) / NW;              // T columns per wave
   constexpr int KO = (2 * NMAX + NXMAX + NW - 1) / NW;  // D, E, B columns per wave
   __shared__ TW colbuf[2][RS * 64];
+  __shared__ TW lbuf[RS * 64], mbuf[RS * 64];
+  __shared__ int sbuf[2];
+  __shared__ int posbuf[128];
   if (p >= a.npairs) return;   // uniform over the workgroup
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -168,51 +171,79 @@ __device__ __forceinline__ void pair_lu_cols_body(FactorArgs<TS, TW> const& a, i
   __syncthreads();
 
   // ---- n elimination steps --------------------------------------------------------
-  int pos0 = lane, pos1 = lane + 64;   // positions of rows lane, lane + 64
+  // Two barriers per step.  Wave 0 alone finds the pivot of column c, keeps
+  // the row positions, forms the multipliers and publishes them (lbuf, with
+  // zeros for the rows not updated), the final column c (mbuf) and the pivot
+  // lane; then every wave updates its columns.  The search and the division
+  // run once per step instead of once per wave: the step loop is bound by
+  // VALU issue, not by its latency, when enough pairs share a CU.
+  int pos0 = lane, pos1 = lane + 64;   // wave 0: positions of rows lane, lane + 64
   int myperm = 0;
   bool zero_pivot = false;
   for (int c = 0; c < n; ++c) {
-    const TW* cur = colbuf[c & 1];
-    TW col[RS];
+    if (w == 0) {
+      const TW* cur = colbuf[c & 1];
+      TW col[RS];
 #pragma unroll
-    for (int s = 0; s < RS; ++s) col[s] = cur[lane + 64 * s];
-    const bool c0 = lane < n2 && pos0 >= c;
-    const bool c1 = RS > 1 && lane + 64 < n2 && pos1 >= c;
-    unsigned h0, l0, h1 = 0, l1 = 0;
-    wave::abs_key(col[0], h0, l0);
-    if (RS > 1) wave::abs_key(col[RS > 1 ? 1 : 0], h1, l1);
-    // the better of this lane's two candidates (ties to the smaller position)
-    const bool b1 = c1 && (!c0 || h1 > h0 || (h1 == h0 && (l1 > l0 || (l1 == l0 && pos1 < pos0))));
-    bool zero;
-    const int who = wave::argmax_pos(c0 || c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane, zero);
-    const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
-    const int sp = __builtin_amdgcn_readlane(int(b1), who);
-    zero_pivot = zero_pivot || zero;
-    if (lane == c) myperm = P + 1;
-    // the pivot row moves to position c, the row at c to the pivot's old place
-    pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
-    pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
-    const TW pv = sp ? wave::rl(col[RS > 1 ? 1 : 0], who) : wave::rl(col[0], who);
-    const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
-    bool upd[RS];
-    TW lv[RS];
+      for (int s = 0; s < RS; ++s) col[s] = cur[lane + 64 * s];
+      const bool c0 = lane < n2 && pos0 >= c;
+      const bool c1 = RS > 1 && lane + 64 < n2 && pos1 >= c;
+      unsigned h0, l0, h1 = 0, l1 = 0;
+      wave::abs_key(col[0], h0, l0);
+      if (RS > 1) wave::abs_key(col[RS > 1 ? 1 : 0], h1, l1);
+      // the better of this lane's two candidates (ties to the smaller
+      // position; bitwise operators: no short-circuit branches)
+      const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
+      bool zero;
+      const int who = wave::argmax_pos<(sizeof(TW) > 4)>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0,
+                                                         lane, zero);
+      const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
+      const int sp = __builtin_amdgcn_readlane(int(b1), who);
+      zero_pivot = zero_pivot || zero;
+      if (lane == c) myperm = P + 1;
+      // the pivot row moves to position c, the row at c to the pivot's old place
+      pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
+      pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+      const TW pv = sp ? wave::rl(col[RS > 1 ? 1 : 0], who) : wave::rl(col[0], who);
+      const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
 #pragma unroll
-    for (int s = 0; s < RS; ++s) {   // rows below the pivot: multiplier, others 0
-      const int r = lane + 64 * s;
-      upd[s] = r < n2 ? (s == 0 ? pos0 > c : pos1 > c) : r < R;
-      lv[s] = upd[s] ? col[s] * rinv : TW(0);
+      for (int s = 0; s < RS; ++s) {   // rows below the pivot: multiplier, others 0
+        const int r = lane + 64 * s;
+        const bool upd = r < n2 ? (s == 0 ? pos0 > c : pos1 > c) : r < R;
+        const TW l = col[s] * rinv;
+        lbuf[lane + 64 * s] = upd ? l : TW(0);
+        mbuf[lane + 64 * s] = upd ? l : col[s];
+      }
+      if (lane == 0) {
+        sbuf[0] = who;
+        sbuf[1] = sp;
+      }
     }
+    __syncthreads();   // pivot, multipliers and column c published
+    TW lv[RS], mc[RS];
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+      lv[s] = lbuf[lane + 64 * s];
+      mc[s] = mbuf[lane + 64 * s];
+    }
+    const int who = __builtin_amdgcn_readfirstlane(sbuf[0]);
+    const int sp = __builtin_amdgcn_readfirstlane(sbuf[1]);
     TW* next = colbuf[(c + 1) & 1];
     // the column guards below compare compile-time k with these; opaque
     // copies keep the compiler from hoisting every guard out of the step loop
     // into its own SGPRs (which then spill and are reloaded each step)
     int w_ = w, n_ = n, no_ = no;
     asm volatile("" : "+s"(w_), "+s"(n_), "+s"(no_));
-    update_columns<NW>(vt, vo, lv, upd, sp != 0, who, w_, c, n_, no_, next, lane);
-    // column c + 1 published; every wave is done reading column c's slot
+    update_columns<NW>(vt, vo, lv, mc, sp != 0, who, w_, c, n_, no_, next, lane);
+    // column c + 1 published; every wave is done with this step's buffers
     __syncthreads();
   }
-  if (zero_pivot && w == 0 && lane == 0) atomicOr(a.status, 1);
+  if (w == 0) {
+    if (zero_pivot && lane == 0) atomicOr(a.status, 1);
+    posbuf[lane] = pos0;
+    if (RS > 1) posbuf[64 + lane] = pos1;
+  }
+  __syncthreads();   // final row positions published
 
   // ---- store every row at its final position -----------------------------------
   TW* Tj = a.Tm + size_t(j) * tsize(n);
@@ -220,7 +251,7 @@ __device__ __forceinline__ void pair_lu_cols_body(FactorArgs<TS, TW> const& a, i
   for (int s = 0; s < RS; ++s) {
     const int r = lane + 64 * s;
     if (r >= R) continue;
-    const int q = r < n2 ? (s == 0 ? pos0 : pos1) : r;
+    const int q = r < n2 ? posbuf[r] : r;
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
       const int col = w + NW * k;

@@ -64,11 +64,14 @@ __device__ __forceinline__ void abs_key(float v, unsigned& hi, unsigned& lo) {
 // LAPACK idamax over the wave: among the lanes with `cand`, the largest key,
 // ties to the smallest `pos`.  Returns the winning lane (uniform); `zero` is
 // set when the largest candidate magnitude is exactly zero.
+// LO = false: the low key word is zero on every lane (FP32 keys) and is not
+// reduced.
+template <bool LO = true>
 __device__ __forceinline__ int argmax_pos(bool cand, unsigned khi, unsigned klo, int pos, int lane, bool& zero) {
   khi = cand ? (khi | 0x80000000u) : 0u;
   klo = cand ? klo : 0u;
   const unsigned hmax = max_u32(khi);
-  const unsigned lmax = max_u32(khi == hmax ? klo : 0u);
+  const unsigned lmax = LO ? max_u32(khi == hmax ? klo : 0u) : 0u;
   const unsigned long long tie = __ballot(khi == hmax && klo == lmax);
   int who;
   if (__popcll(tie) == 1) {
