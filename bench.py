@@ -228,6 +228,11 @@ def bench_batched(args):
     mix.plan(dims, operator=False)
     mix.share_operator(ref)
     L = ref.size("x")
+    if args.calibrate_copy:   # a kernel of known traffic (1 GiB read + 1 GiB written) for counter scaling
+        ca, cb = DeviceArray(1 << 27), DeviceArray(1 << 27)
+        cb.copy_from(ca)
+        ca.free()
+        cb.free()
     B = DeviceArray.from_host(np.random.default_rng(args.seed + 5).standard_normal(L))
     X = DeviceArray(L)
     print(f"[bench] {nsys} systems generated on the GPU in {time.perf_counter() - t_gen:.1f} s "
@@ -263,6 +268,13 @@ def bench_batched(args):
         out["mixed_without_selector"] = dict(solves_per_s=nsys / t_none, ms=1e3 * t_none, fp64_fallbacks=fb_none,
                                              max_sweeps=r_none["max_sweeps"],
                                              max_rel_residual=r_none["max_rel_residual"])
+        # without the FP64 safety net: how many systems miss the tolerance
+        # with FP32 LU everywhere and with the selector's LU/QRP choice
+        mix.solve_selected(B, X, tol, 30, selector=None, fallback=False)
+        f_lu = int(mix.report()["failed"])
+        mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=False)
+        f_sel = int(mix.report()["failed"])
+        out["without_fallback"] = dict(failed_fp32_lu_only=f_lu, failed_with_selector=f_sel)
         out["selector"] = dict(sent_to_qrp=int(chosen["chose_qrp"].sum()),
                                fp64_fallbacks=int(chosen["fell_back"].sum()),
                                fp64_lu=int((chosen["fallback_stage"] == 1).sum()),
