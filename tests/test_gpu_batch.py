@@ -116,3 +116,51 @@ def test_batch_rejects_rectangular_last_block():
     bt = Batch(mixed=False)
     with pytest.raises(AlglinError):
         bt.plan([(10, 4, 2, 1, 1, 1, 0)])
+
+
+def test_time_factorize_covers_every_launch():
+    """alglin_batch_time_factorize: one row per (level, kernel class) launch
+    of one factorization, each with a positive duration and the algorithmic
+    bytes and flops of the pairs it eliminates; the per-launch bytes add up
+    to the whole batch's pair bytes (FP32 factors, FP64 level-0 reads)."""
+    dims = np.asarray([(N, n, n // 4, n // 4, 2, 2, 0) for N, n in
+                       [(300, 8), (300, 20), (200, 40), (37, 4), (100, 33)]], np.int32)
+    bt = Batch(mixed=True)
+    bt.plan(dims)
+    bt.generate(seed=5, shift=4.0)
+    rows = bt.time_factorize()
+    levels = int(np.ceil(np.log2(dims[:, 0].max())))
+    assert rows.shape[1] == 6 and len(rows) >= levels
+    assert set(np.unique(rows[:, 0]).astype(int)) == set(range(levels))
+    assert np.all(rows[:, 3] > 0) and np.all(rows[:, 4] > 0) and np.all(rows[:, 5] > 0)
+    want = 0.0
+    for N, n, qr, qx, nr, nx, _ in dims:
+        L = 1
+        while L < N:
+            p = (N + L - 1) // (2 * L)
+            rd = 4 * n * n + 2 * n * nx + 2 * nr * n
+            wr = 6 * n * n + 2 * n * nx + 3 * nr * n + nr * nx
+            want += p * (rd * (8 if L == 1 else 4) + wr * 4 + 4 * n)
+            L *= 2
+    assert abs(rows[:, 4].sum() - want) <= 1e-9 * want
+
+
+def test_selected_fallback_stages_reach_tolerance():
+    """solve_selected with the FP64 fallback on systems with nearly singular
+    pairs: every system ends at the tolerance; the ones FP32 LU + refinement
+    cannot bring there go to plain FP64 LU (stage 1), and only what that
+    misses to FP64 QRP (stage 2)."""
+    from alglin_amd import selector as S
+    bt, B, sing = S.make_batch(512, seed=91, Nmin=64, Nmax=512)
+    X = DeviceArray(bt.size("x"))
+    rc = bt.solve_selected(B, X, tol=1e-12, selector=None, fallback=True)
+    rep = bt.report()
+    sel = bt.selection()
+    print(f"stages {np.bincount(sel['fallback_stage'], minlength=3)} failed {rep['failed']}", flush=True)
+    assert rc == 0 and rep["failed"] == 0
+    assert np.all(rep["rel_residual"] <= 1e-12)
+    st = sel["fallback_stage"]
+    assert set(np.unique(st)) <= {0, 1, 2}
+    assert (st >= 1).sum() > 0
+    # the systems that need the fallback are mostly the nearly singular ones
+    assert (st[sing >= 1] >= 1).sum() > (st[sing < 1] >= 1).sum()
