@@ -555,7 +555,8 @@ typename BatchEngine<TA, TW>::MvTables& BatchEngine<TA, TW>::mv_tables(double co
     m.x = X + off_[kbX][s]; m.b = B ? B + off_[kbX][s] : nullptr; m.out = R + off_[kbX][s];
     m.Cpart = Cpart_.p + offCpart_[s];
     m.skip = skip_.p + s;
-    const int blocks = (d.N + 1 + 3) / 4;
+    const int per_block = 4 * mv_rows_per_wave(d.n, d.nr);   // block rows per 256-thread workgroup
+    const int blocks = (d.N + 1 + per_block - 1) / per_block;
     for (int b = 0; b < blocks; ++b) entry.push_back(i);
     first.push_back(first.back() + blocks);
     ReduceArgs<double, double>& r = cs[i];

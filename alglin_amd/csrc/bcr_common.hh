@@ -114,6 +114,14 @@ struct MvArgs {
 };
 
 __host__ __device__ __forceinline__ size_t tsize(int n) { return size_t(2) * n * n + n; }
+// block rows one wave of the residual SpMV takes: a block row uses
+// g = max(n, nr) lanes (its n rows and its nr border partials), 64 / g rows
+// per wave for g <= 32
+__host__ __device__ __forceinline__ int mv_group(int n, int nr) { return n > nr ? n : (nr > 0 ? nr : 1); }
+__host__ __device__ __forceinline__ int mv_rows_per_wave(int n, int nr) {
+  const int g = mv_group(n, nr);
+  return g <= 32 ? 64 / g : 1;
+}
 __host__ __device__ __forceinline__ size_t align16(size_t bytes) { return (bytes + 15) & ~size_t(15); }
 
 template <typename T> __device__ __forceinline__ T dev_abs(T v) { return v < T(0) ? -v : v; }

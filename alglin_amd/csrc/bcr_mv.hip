@@ -16,75 +16,101 @@ namespace bcr {
 
 // Loads are issued eight columns at a time from clamped addresses and masked
 // after (keep_if), so each chunk costs one memory latency instead of one per
-// column; the sums run in the original column order.
-template <typename TA>
-__device__ __forceinline__ void mv_rows_body(MvArgs<TA> a, int bid) {
+// column; the sums run in the original column order.  A block row uses
+// g = max(n, nr) lanes; for g <= 32 a wave takes 64 / g block rows (lane =
+// sub-row * g + row), so small blocks do not leave most lanes idle; x is held one entry per lane and broadcast per
+// column (v_readlane with one block row per wave, ds_bpermute within a
+// sub-row group otherwise).
+template <typename TA, bool PACKED>
+__device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
   constexpr int CH = 8;
   const int lane = threadIdx.x & 63;
-  // wave-uniform block row (made explicit: the row's addresses stay scalar)
-  const int k = __builtin_amdgcn_readfirstlane(bid * 4 + int(threadIdx.x >> 6));
   const int N = a.N, n = a.n, nx = a.nx, nr = a.nr;
-  if (k > N) return;
-  const double* xk = a.x + size_t(k) * n;
+  const int g = PACKED ? mv_group(n, nr) : 64;   // lanes per block row
+  const int bpw = PACKED ? 64 / g : 1;
+  const int k0 = __builtin_amdgcn_readfirstlane((bid * 4 + int(threadIdx.x >> 6)) * bpw);
+  if (k0 > N) return;
+  const int sub = PACKED ? lane / g : 0, r = PACKED ? lane - sub * g : lane;
+  const int k = k0 + sub;                       // this lane's block row
+  const bool live = sub < bpw && k <= N;
+  const int kc = live ? k : k0;                 // clamped: every address stays valid
+  const int base = PACKED ? sub * g : 0;        // lane holding column 0 of this sub-row
+  auto bcast = [&](double v, int c) -> double {
+    if constexpr (PACKED) return __shfl(v, base + c, 64);
+    else return wave::rl(v, c);
+  };
+  const double* xk = a.x + size_t(kc) * n;
   const double* xb = a.x + size_t(N + 1) * n + a.qx;
-  // x_k (and x_{k+1} below) one entry per lane (n <= 64), broadcast per
-  // column by v_readlane instead of one vector load per column
-  const double xa = lane < n ? xk[lane] : 0.0;
-  if (k < N) {
-    const TA* Dk = a.D + size_t(k) * n * n;
-    const TA* Ek = a.E + size_t(k) * n * n;
-    const TA* Bk = a.B + size_t(k) * n * nx;
-    const double xn = lane < n ? xk[n + lane] : 0.0;
-    for (int r = lane; r < n; r += 64) {
-      double acc = 0;
-      for (int c0 = 0; c0 < n; c0 += CH) {
-        TA d[CH], e[CH];
+  const double xa = r < n ? xk[r < n ? r : 0] : 0.0;   // x_k, one entry per lane
+  if (k0 < N) {
+    const bool row = live && k < N && r < n;
+    const int kr = row ? k : k0;
+    const TA* Dk = a.D + size_t(kr) * n * n;
+    const TA* Ek = a.E + size_t(kr) * n * n;
+    const TA* Bk = a.B + size_t(kr) * n * nx;
+    const int rr = r < n ? r : 0;
+    const double xn = (row ? xk : a.x + size_t(k0) * n)[n + rr];   // x_{k+1}
+    double acc = 0;
+    for (int c0 = 0; c0 < n; c0 += CH) {
+      TA d[CH], e[CH];
 #pragma unroll
-        for (int i = 0; i < CH; ++i) {
-          const int cc = c0 + i < n ? c0 + i : 0;
-          d[i] = Dk[r + cc * n];
-          e[i] = Ek[r + cc * n];
-        }
-#pragma unroll
-        for (int i = 0; i < CH; ++i) {
-          const int cc = c0 + i < n ? c0 + i : 0;
-          const double u = wave::rl(xa, cc), v = wave::rl(xn, cc);
-          acc += keep_if(double(d[i]) * u + double(e[i]) * v, c0 + i < n);
-        }
+      for (int i = 0; i < CH; ++i) {
+        const int cc = c0 + i < n ? c0 + i : 0;
+        d[i] = Dk[rr + cc * n];
+        e[i] = Ek[rr + cc * n];
       }
-      for (int c0 = 0; c0 < nx; c0 += CH) {
-        TA bb[CH];
-        double w[CH];
 #pragma unroll
-        for (int i = 0; i < CH; ++i) {
-          const int cc = c0 + i < nx ? c0 + i : 0;
-          bb[i] = Bk[r + cc * n];
-          w[i] = xb[cc];
-        }
-#pragma unroll
-        for (int i = 0; i < CH; ++i) acc += keep_if(double(bb[i]) * w[i], c0 + i < nx);
+      for (int i = 0; i < CH; ++i) {
+        const int cc = c0 + i < n ? c0 + i : 0;
+        const double u = bcast(xa, cc), v = bcast(xn, cc);
+        acc += keep_if(double(d[i]) * u + double(e[i]) * v, c0 + i < n);
       }
-      const size_t row = size_t(k) * n + r;
-      a.out[row] = a.b ? a.b[row] - acc : acc;
+    }
+    for (int c0 = 0; c0 < nx; c0 += CH) {
+      TA bb[CH];
+      double w[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int cc = c0 + i < nx ? c0 + i : 0;
+        bb[i] = Bk[rr + cc * n];
+        w[i] = xb[cc];
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) acc += keep_if(double(bb[i]) * w[i], c0 + i < nx);
+    }
+    if (row) {
+      const size_t out = size_t(k) * n + r;
+      a.out[out] = a.b ? a.b[out] - acc : acc;
     }
   }
-  const TA* Ck = a.C + size_t(k) * nr * n;
-  for (int i = lane; i < nr; i += 64) {
+  // border partials C_k x_k: lanes (sub-row, i) with i < nr
+  if (nr > 0) {
+    const int i = r;
+    const bool crow = live && i < nr;
+    const TA* Ck = a.C + size_t(kc) * nr * n;
+    const int ic = crow ? i : 0;
     double acc = 0;
     for (int c0 = 0; c0 < n; c0 += CH) {
       TA cv[CH];
-      double u[CH];
 #pragma unroll
       for (int t = 0; t < CH; ++t) {
         const int cc = c0 + t < n ? c0 + t : 0;
-        cv[t] = Ck[i + cc * nr];
-        u[t] = wave::rl(xa, cc);
+        cv[t] = Ck[ic + cc * nr];
       }
 #pragma unroll
-      for (int t = 0; t < CH; ++t) acc += keep_if(double(cv[t]) * u[t], c0 + t < n);
+      for (int t = 0; t < CH; ++t) {
+        const int cc = c0 + t < n ? c0 + t : 0;
+        acc += keep_if(double(cv[t]) * bcast(xa, cc), c0 + t < n);
+      }
     }
-    a.Cpart[size_t(k) * nr + i] = acc;
+    if (crow) a.Cpart[size_t(k) * nr + i] = acc;
   }
+}
+
+template <typename TA>
+__device__ __forceinline__ void mv_rows_body(MvArgs<TA> const& a, int bid) {
+  if (mv_rows_per_wave(a.n, a.nr) > 1) mv_rows_run<TA, true>(a, bid);
+  else mv_rows_run<TA, false>(a, bid);
 }
 
 template <typename TA>
@@ -192,7 +218,8 @@ __global__ void __launch_bounds__(256) sumsq_final_kernel(const double* __restri
 
 template <typename TA>
 hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st) {
-  hipLaunchKernelGGL((mv_rows_kernel<TA>), dim3((a.N + 1 + 3) / 4), dim3(256), 0, st, a);
+  const int rows_per_block = 4 * mv_rows_per_wave(a.n, a.nr);
+  hipLaunchKernelGGL((mv_rows_kernel<TA>), dim3((a.N + 1 + rows_per_block - 1) / rows_per_block), dim3(256), 0, st, a);
   if (a.nr > 0) {
     launch_convert<double, double>(nullptr, Csum, a.nr, st);
     launch_reduce<double, double>(a.Cpart, 0, a.N + 1, a.nr, tmp, max_chunks, Csum, a.nr, a.nr, st);
