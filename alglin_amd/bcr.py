@@ -72,6 +72,7 @@ def lib():
             "alglin_memcpy_dtod": (ci, [vp, vp, i64]),
             "alglin_memcpy_dtod_async": (ci, [vp, vp, i64, vp]),
             "alglin_batch_time_factorize": (ci, [vp, vp, i32]),
+            "alglin_batch_set_pair_kernel": (ci, [vp, ctypes.c_int]),
             "alglin_batch_create": (vp, [ctypes.c_int]),
             "alglin_batch_destroy": (None, [vp]),
             "alglin_batch_plan": (ci, [vp, i32, vp]),
@@ -272,7 +273,8 @@ class BorderedCR:
 
     def set_pair_kernel(self, variant: int):
         """LU pair kernel: 0 default, 2 two-role kernel (16 < n <= 32), 3 register tiles for every n,
-        4 column-split waves for every n <= 64."""
+        4 column-split waves for every n <= 64, 5 matrix-core right-hand update (FP32 factors,
+        n <= 48, n + nr <= 64, nx <= 16)."""
         self._check(lib().alglin_bcr_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def set_fallback(self, on: bool):
@@ -558,6 +560,12 @@ class Batch:
 
     def stream(self) -> int:
         return lib().alglin_batch_stream(self._h)
+
+    def set_pair_kernel(self, variant: int):
+        """Kernel of the 32 < n <= 64 LU pair class: 0 default (the matrix-core
+        kernel where it applies, else column-split waves), 4 column-split waves,
+        5 matrix-core right-hand update (FP32 factors, n <= 48, nr <= 8)."""
+        self._check(lib().alglin_batch_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def time_factorize(self) -> np.ndarray:
         """One eager factorization with every pair-elimination launch timed

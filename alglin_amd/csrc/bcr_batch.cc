@@ -8,9 +8,21 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 namespace bcr {
 namespace {
+
+// the matrix-core kernel of the 32 < n <= 48 LU class (FP32 factors only):
+// the default (variant 0) and variant 5; variant 4 keeps the column-split
+// kernel
+template <typename TS, typename TW, typename L>
+bool try_mfma(int variant, BatchTable<FactorArgs<TS, TW>> const& t, L const& l, hipStream_t st, hipError_t* e) {
+  if constexpr (std::is_same_v<TW, float>) {
+    if (variant == 0 || variant == 5) return launch_pair_lu_mfma_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st, e);
+  }
+  return false;
+}
 
 // host staging of the launch tables; uploaded once as one device blob
 struct Blob {
@@ -366,6 +378,8 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
         bool ok = true;
         if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
         else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
+        else if (l.cls == 4 && try_mfma(pair_variant_, t, l, st_, &e))
+          ok = true;
         else if (l.cls == 4 && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
           ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
@@ -756,6 +770,12 @@ void BatchSolver::bind_device(double const* D, double const* E, double const* B,
 void BatchSolver::generate(uint64_t seed, double shift, std::vector<int32_t> const& sing_k, double eps) {
   if (mixed_) e32_->generate(seed, shift, sing_k, eps);
   else e64_->generate(seed, shift, sing_k, eps);
+}
+
+void BatchSolver::set_pair_variant(int v) {
+  drop_graphs();
+  if (e32_) e32_->set_pair_variant(v);
+  if (e64_) e64_->set_pair_variant(v);
 }
 
 void BatchSolver::set_methods(std::vector<int32_t> const& m) {

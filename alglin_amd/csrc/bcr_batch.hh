@@ -96,6 +96,10 @@ class BatchEngine {
   // per-system flags (device, nsys): the residual, norms, solve and add
   // launches skip a system whose flag is nonzero; factorize_async clears them
   int32_t* skip_flags() { return skip_.p; }
+  // kernel of the 32 < n <= 64 LU class: 0 default (the matrix-core kernel
+  // where it applies: FP32 factors, n <= 48; else column-split waves), 4
+  // column-split waves, 5 matrix-core right-hand update
+  void set_pair_variant(int v) { pair_variant_ = v; }
 
   // one eager factorization with HIP events around every pair-elimination
   // launch: per launch its level, kernel class, workgroups, duration and
@@ -108,6 +112,7 @@ class BatchEngine {
   std::vector<LaunchTime> time_factorize();
 
  private:
+  int pair_variant_ = 0;
   struct Launch {
     const void* args;
     const int32_t* entry;
@@ -213,6 +218,8 @@ class BatchSolver {
   std::vector<char> const& fell_back() const { return fell_back_; }
   // per-system methods for solve(): kLU / kQR / kQRP
   void set_methods(std::vector<int32_t> const& m);
+  // kernel of the 32 < n <= 64 LU class (BatchEngine::set_pair_variant)
+  void set_pair_variant(int v);
   std::vector<int32_t> const& status() const { return status_; }
   std::vector<double> const& rel_residual() const { return rel_; }
   std::vector<int32_t> const& sweeps() const { return sweeps_; }

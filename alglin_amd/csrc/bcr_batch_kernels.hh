@@ -116,6 +116,9 @@ bool launch_pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> const& t, int n
 template <typename TS, typename TW>
 bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err);
+template <typename TS>
+bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
+                                 hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 bool launch_pair_lu_tw_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                hipStream_t st, hipError_t* err);

@@ -3,11 +3,21 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "bcr_launch.hh"
 
 namespace bcr {
+
+namespace {
+// the matrix-core pair kernel exists for FP32 factors only
+template <typename TS, typename TW>
+bool launch_mfma(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* e) {
+  if constexpr (std::is_same_v<TW, float>) return launch_pair_lu_mfma(a, st, e);
+  return false;
+}
+}  // namespace
 
 void hip_check(hipError_t e, char const* what) {
   if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
@@ -170,12 +180,14 @@ void Engine<TA, TW>::level_factor(int L, bool first) {
     hipError_t e = hipSuccess;
     // variants: 0 default (wave-per-row kernels for n <= 32, column-split
     // waves up to n = 64, register tiles beyond), 2 two-role (16 < n <= 32),
-    // 3 register tiles only, 4 column-split waves only
+    // 3 register tiles only, 4 column-split waves only, 5 matrix-core
+    // right-hand update (FP32 factors, n <= 48)
     const int v = pair_variant_;
     const bool lu_fast =
         method_ == kLU &&
         ((v == 2 && launch_pair_lu_tw(a, st_, &e)) || (v == 4 && launch_pair_lu_cols(a, st_, &e)) ||
-         (v != 3 && v != 4 &&
+         (v == 5 && launch_mfma(a, st_, &e)) ||
+         (v != 3 && v != 4 && v != 5 &&
           (launch_pair_lu_split(a, st_, &e) || launch_pair_lu_fast(a, st_, &e) || launch_pair_lu_cols(a, st_, &e))) ||
          launch_pair_lu_tiled(a, st_, &e));
     if (lu_fast) {

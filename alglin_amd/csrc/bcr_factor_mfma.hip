@@ -1,0 +1,484 @@
+// Pair elimination, LU, FP32 factors, n <= 48: the getrf + applyT split of
+// the reference pair step (buildT BABD_BorderedCR.cc:856, applyT cc:917-982,
+// the border update cc:1107-1154), with the right-hand columns on the matrix
+// cores.  The augmented panel is the one of bcr_factor_generic.hip: R = 2n +
+// nr rows [row jp; row j; border], columns [T = col j | col jp | col jpp | xb].
+// One wave per pair (the LDS a pair needs, 31 KB, then admits five pairs per
+// CU; the steps are latency-bound, so pairs in flight are what counts).
+//
+//  1. The T panel [E_jp; D_j; C_j] (R x n) in registers, rows over lanes
+//     (row lane + 64 s, s < 2), columns in 16-wide register vectors: LAPACK
+//     partial pivoting among the first 2n rows (first maximum), multiplier
+//     a * (1/pivot), fma updates -- the arithmetic of the other LU kernels,
+//     so T, the pivots and C_j U^-1 agree with them bit for bit.  The step
+//     loop is rolled within each 16-column block (column c read and written
+//     through a uniform register index), a few KB of code.  Rows do not
+//     move; their final positions are tracked.  T, perm and C_j U^-1 go to
+//     their blocks, and LDS gets
+//        Ls[0, NT)        L11^-1 (unit lower, forward substitution, one
+//                         column per lane, in place over L11)
+//        Ls[NT, NT + NR)  -[L21; C_j U^-1] (final positions n .. 2n + nr - 1)
+//  2. The right-hand part X = [col jp | col jpp | xb] in 16-column tiles,
+//     each inside one of the three column blocks.  A tile's sources are
+//     loaded one tile ahead, staged in LDS with the rows at their final
+//     positions (column-major, NT + NR slots per column), and
+//        U12     = L11^-1 X_top             -> pivot rows: D_j', E_j', B_j'
+//        X_rest' = X_rest - [L21; C_j U^-1] U12
+//                                           -> reduced rows D_jp', E_jp', B_jp'
+//     run on v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation), the
+//     row tiles' independent accumulators alternating.  The sum over k runs
+//     as 16-wide blocks, lane group g = lane / 16 taking k = 16 kb + 4 g + i
+//     in the i-th MFMA of a block, so that every operand is one 16-byte LDS
+//     read per lane per four MFMAs.  The border rows (C_jp', pending C_jpp,
+//     F partial) leave through LDS in one coalesced pass per tile.
+// The right-hand results differ from the rank-1 kernels in rounding only
+// (other summation order, the explicit inverse of L11).  A pivot whose
+// reciprocal overflows (a subnormal pivot) turns the finished T columns into
+// NaN here, where the rank-1 kernels leave them; the factorization is
+// unusable either way and the refinement reports it.
+#include "bcr_launch.hh"
+#include "bcr_batch_kernels.hh"
+#include "bcr_wave.hh"
+
+namespace bcr {
+namespace {
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// LDS geometry: NT top slots (pivot positions, NMAX a multiple of 16), NR
+// rest slots (reduced and border rows, n + nr <= RESTMAX)
+template <int NMAX, int RESTMAX>
+struct MfmaGeom {
+  static constexpr int NT = NMAX;
+  static constexpr int NR = (RESTMAX + 15) / 16 * 16;
+  static constexpr int LD = NT + 4;        // row stride of Ls
+  static constexpr int XLD = NT + NR + 4;  // column stride of a staged tile
+  static_assert(NT % 16 == 0, "NMAX must be a multiple of 16");
+  static_assert(NT + NR <= 128, "padding slots come from the lanes beyond row R");
+};
+
+__device__ __forceinline__ f4_t mfma4(f4_t a, f4_t b, f4_t c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
+  return c;
+}
+
+// lane `who`'s value of v when K > cc, else 0: a scalar select on the
+// v_readlane result (written out: the compiler would move the value to a
+// VGPR and select there, two vector instructions instead of none)
+template <int K>
+__device__ __forceinline__ float rl_after(float v, int who, int cc) {
+  int x = __builtin_amdgcn_readlane(__float_as_int(v), who);
+  asm("s_cmp_lt_u32 %1, %2\n\ts_cselect_b32 %0, %0, 0" : "+s"(x) : "s"(cc), "n"(K) : "scc");
+  return __int_as_float(x);
+}
+
+// columns K.. 15 of one 16-column block of both row sets, those not after
+// column cc of the block getting a zero pivot-row entry
+template <int K>
+__device__ __forceinline__ void update_after(f16v& v0, f16v& v1, bool sp, int who, int cc, float lv0, float lv1) {
+  if constexpr (K < 16) {
+    const float u = rl_after<K>(sp ? v1[K] : v0[K], who, cc);
+    v0[K] = fma(-lv0, u, v0[K]);
+    v1[K] = fma(-lv1, u, v1[K]);
+    update_after<K + 1>(v0, v1, sp, who, cc, lv0, lv1);
+  }
+}
+
+// A panel row of the right-hand part: its class (1: a row of block jp, 2: a
+// row of block j, 4: a border row, 0: none) and its row in that block.
+struct RowSrc {
+  int cls, off;
+};
+
+// the raw source values of one row in right-hand tile t (16 columns of the
+// tile's region; clamped addresses, masked when staged)
+template <typename TS>
+__device__ __forceinline__ void load_rhs_row(TS (&xs)[16], FactorArgs<TS, float> const& a, int t, int tA, int j,
+                                             int jp, RowSrc rs) {
+  const int n = a.n, nr = a.nr, nx = a.nx;
+  const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
+  const int reg = t < tA ? 0 : t < 2 * tA ? 1 : 2;
+  const int c0 = 16 * (reg == 0 ? t : reg == 1 ? t - tA : t - 2 * tA);
+  const int width = reg == 2 ? nx : n;
+  // region 0: [D_jp | . | C_jp]; region 1: [. | E_j | .]; region 2: [B_jp | B_j | .]
+  const bool valid = reg == 0 ? (rs.cls & 5) != 0 : reg == 1 ? rs.cls == 2 : (rs.cls & 3) != 0;
+  const TS* src = reg == 0 ? (rs.cls == 4 ? a.sC + jp * rn : a.sD + jp * nn)
+                : reg == 1 ? a.sE + j * nn
+                           : a.sB + (rs.cls == 1 ? jp : j) * bn;
+  src = valid ? src + rs.off : a.sD + jp * nn;   // a valid address for the rows of zeros
+  const int stride = reg == 0 && rs.cls == 4 ? nr : n;
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) xs[cc] = src[(c0 + cc < width ? c0 + cc : width - 1) * stride];
+}
+
+template <typename TS, int NB, int RESTMAX, int NW>
+__device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a, int p) {
+  using TW = float;
+  constexpr int NMAX = 16 * NB;
+  using G = MfmaGeom<NMAX, RESTMAX>;
+  constexpr int NT = G::NT, NR = G::NR, LD = G::LD, XLD = G::XLD;
+  __shared__ __align__(16) TW Ls[(NT + NR) * LD];
+  __shared__ __align__(16) TW Xt[NW][16 * XLD];
+  __shared__ int posbuf[128];
+  if (p >= a.npairs) return;   // uniform over the workgroup
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int n = a.n, nr = a.nr, nx = a.nx, n2 = 2 * n;
+  const int R = n2 + nr;
+  int j, jp, jpp;
+  pair_index(p, a.L, a.N, j, jp, jpp);
+  const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
+  TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
+  TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;
+
+  if (w == 0) {
+    // ---- 1. the T panel, 16-column register vectors --------------------------------
+    // the border rows C_j (+ pending C_j) first, coalesced, into LDS rows of
+    // NMAX (row nr stays zero for the other lanes)
+    TW* stage = Xt[0];
+    for (int e = lane; e < (nr + 1) * NMAX; e += 64) stage[e] = TW(0);
+    for (int e = lane; e < nr * n; e += 64) {   // C_j is nr x n, column-major
+      const int col = e / nr, ri = e - col * nr;
+      TW x = TW(a.sC[j * rn + e]);
+      if (!a.first) x += pend_prv[j * rn + e] + pend_cur[j * rn + e];
+      stage[ri * NMAX + col] = x;
+    }
+    f16v V[2][NB];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int r = lane + 64 * s;
+      const bool top = r < n, mid = r >= n && r < n2, bord = r >= n2 && r < R;
+      // rows of block jp: E_jp; rows of block j: D_j
+      const TS* base = top ? a.sE + jp * nn + r : a.sD + j * nn + (mid ? r - n : 0);
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const int col = 16 * b + k, off = (col < n ? col : n - 1) * n;
+          V[s][b][k] = keep_if(TW(base[off]), (top || mid) && col < n);
+        }
+      if (R > 64 * s && n2 < 64 * (s + 1)) {   // this row set holds border rows
+        const TW* srow = stage + (bord ? r - n2 : nr) * NMAX;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int k = 0; k < 16; k += 4) {
+            const f4_t c4 = *reinterpret_cast<const f4_t*>(srow + 16 * b + k);
+            V[s][b][k] += c4.x;
+            V[s][b][k + 1] += c4.y;
+            V[s][b][k + 2] += c4.z;
+            V[s][b][k + 3] += c4.w;
+          }
+      }
+    }
+    int pos0 = lane, pos1 = lane + 64;   // positions of rows lane, lane + 64
+    int myperm = 0;
+    bool zero_pivot = false;
+    // Steps in blocks of 16: step c of block b reads and rewrites column c
+    // through a uniform register index (s_set_gpr_idx) and updates the
+    // columns of blocks b.. with the pivot row's entries, those of block b
+    // not after c getting a zero entry.  The loop stays rolled: a few KB of
+    // code instead of one copy per step.
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int cend = min(16 * b + 16, n);
+      for (int c = 16 * b; c < cend; ++c) {
+        const int cc = c - 16 * b;
+        const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
+        const bool c0 = lane < n2 && pos0 >= c;
+        const bool c1 = lane + 64 < n2 && pos1 >= c;
+        unsigned h0, l0, h1, l1;
+        wave::abs_key(x0, h0, l0);
+        wave::abs_key(x1, h1, l1);
+        // the better of this lane's two candidates (ties to the smaller position)
+        const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
+        bool zero;
+        const int who = wave::argmax_pos<false>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane, zero);
+        const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
+        const bool sp = __builtin_amdgcn_readlane(int(b1), who) != 0;
+        zero_pivot = zero_pivot || zero;
+        myperm = lane == c ? P + 1 : myperm;
+        // the pivot row moves to position c, the row at c to the pivot's old place
+        pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
+        pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+        const TW pv = wave::rl(sp ? x1 : x0, who);
+        const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
+        // rows below the pivot get the multiplier, the others keep column c
+        const bool u0 = lane < n2 ? pos0 > c : lane < R;
+        const bool u1 = lane + 64 < n2 ? pos1 > c : lane + 64 < R;
+        const TW m0 = x0 * rinv, m1 = x1 * rinv;
+        const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
+        V[0][b][cc] = u0 ? m0 : x0;
+        V[1][b][cc] = u1 ? m1 : x1;
+        // columns 1..15 of the block (column 0 is never after c)
+        update_after<1>(V[0][b], V[1][b], sp, who, cc, lv0, lv1);
+#pragma unroll
+        for (int bb = b + 1; bb < NB; ++bb)
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            const TW u = wave::rl(sp ? V[1][bb][k] : V[0][bb][k], who);
+            V[0][bb][k] = fma(-lv0, u, V[0][bb][k]);
+            V[1][bb][k] = fma(-lv1, u, V[1][bb][k]);
+          }
+      }
+    }
+    if (zero_pivot && lane == 0) atomicOr(a.status, 1);
+    posbuf[lane] = pos0;
+    posbuf[64 + lane] = pos1;
+
+    // T (2n x n, final row order) and C_j U^-1 to their blocks; L to LDS
+    TW* Tj = a.Tm + size_t(j) * tsize(n);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int r = lane + 64 * s;
+      const int q = r < n2 ? (s == 0 ? pos0 : pos1) : r;
+      if (r < R) {
+        TW* dst = q < n2 ? Tj + q : a.C + j * rn + (q - n2);
+        const int stride = q < n2 ? n2 : nr;
+        TW* lrow = q < n ? Ls + q * LD : Ls + (NT + q - n) * LD;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            const int col = 16 * b + k;
+            const TW v = V[s][b][k];
+            if (col < n) dst[size_t(col) * stride] = v;
+            // top rows: the strict lower part of L11; other rows: minus the multipliers
+            lrow[col] = q < n ? (col < q ? v : TW(0)) : (col < n ? -v : TW(0));
+          }
+      } else {
+        const int i = r - R;   // rest padding rows n + nr .. NR - 1 to zero
+        if (i < NR - (n + nr)) {
+          TW* lrow = Ls + (NT + n + nr + i) * LD;
+#pragma unroll
+          for (int k = 0; k < NMAX; ++k) lrow[k] = TW(0);
+        }
+      }
+    }
+    if (lane < n) a.perm[size_t(j) * n + lane] = myperm;
+
+    // L11^-1 in place, column `lane`: y_i = -sum_{k<i} L_ik y_k, y_lane = 1;
+    // rows beyond n are identity rows (their slots hold zeros)
+    TW y[NMAX];
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) y[i] = TW(0);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) {
+      // four partial sums: four short dependency chains instead of one
+      TW t0 = TW(0), t1 = TW(0), t2 = TW(0), t3 = TW(0);
+      if (i < n) {
+        const TW* li = Ls + i * LD;
+#pragma unroll
+        for (int k0 = 0; k0 < i; k0 += 4) {
+          const f4_t l4 = *reinterpret_cast<const f4_t*>(li + k0);
+          t0 = fma(l4.x, y[k0], t0);
+          if (k0 + 1 < NMAX) t1 = fma(l4.y, y[k0 + 1 < NMAX ? k0 + 1 : 0], t1);
+          if (k0 + 2 < NMAX) t2 = fma(l4.z, y[k0 + 2 < NMAX ? k0 + 2 : 0], t2);
+          if (k0 + 3 < NMAX) t3 = fma(l4.w, y[k0 + 3 < NMAX ? k0 + 3 : 0], t3);
+        }
+      }
+      y[i] = lane == i ? TW(1) : -((t0 + t1) + (t2 + t3));
+      if (lane < NT) Ls[i * LD + lane] = y[i];
+    }
+  }
+  // T stored, L11^-1 and the rest multipliers in LDS, final positions known
+  __syncthreads();
+
+  // ---- 2. right-hand tiles on the matrix cores ------------------------------------
+  // Tiles lie inside one column block of the right-hand part (region 0: col
+  // jp, region 1: col jpp, both n wide; region 2: xb, nx wide), so the row
+  // sources and destinations are fixed per tile.
+  TW* xt = Xt[w];
+  const int g = lane >> 4, cl = lane & 15;
+  int slot[2];
+  RowSrc rsrc[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int r = lane + 64 * s;
+    rsrc[s].cls = r < n ? 1 : r < n2 ? 2 : r < R ? 4 : 0;
+    rsrc[s].off = r < n ? r : r < n2 ? r - n : r < R ? r - n2 : 0;
+    if (r < n2) {
+      const int q = posbuf[r];
+      slot[s] = q < n ? q : NT + q - n;
+    } else if (r < R) {
+      slot[s] = NT + r - n;
+    } else {   // lanes beyond the panel zero the padding slots
+      const int i = r - R;
+      slot[s] = i < NT - n ? n + i : i - (NT - n) < NR - (n + nr) ? NT + n + nr + (i - (NT - n)) : NT + NR;
+    }
+  }
+  const int tA = (n + 15) / 16, tC = (nx + 15) / 16;
+  const int ntiles = 2 * tA + tC;
+  // the tile's raw source values, loaded one tile ahead: row lane + 64 s,
+  // column c0 + cc of the tile's region
+  TS xs[2][16];
+  load_rhs_row(xs[0], a, w, tA, j, jp, rsrc[0]);
+  load_rhs_row(xs[1], a, w, tA, j, jp, rsrc[1]);
+  for (int t = w; t < ntiles; t += NW) {
+    const int reg = t < tA ? 0 : t < 2 * tA ? 1 : 2;
+    const int c0 = 16 * (reg == 0 ? t : reg == 1 ? t - tA : t - 2 * tA);
+    const int width = reg == 2 ? nx : n;
+    // stage the tile: column cc at xt[cc * XLD + slot], zeros outside the region's rows
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int cls = rsrc[s].cls;
+      const bool valid = reg == 0 ? (cls & 5) != 0 : reg == 1 ? cls == 2 : (cls & 3) != 0;
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = keep_if(TW(xs[s][cc]), valid && c0 + cc < width);
+    }
+    // the border rows' extra terms, nr x 16 contiguous values each (element
+    // e = border row e % nr, tile column e / nr): pending C_jp (region 0, read
+    // once, cleared for reuse) and the pending C_jpp this tile adds to (region 1)
+    const int m = nr * min(16, width - c0);
+    const size_t eb = size_t(c0) * nr;
+    TW pin[2] = {TW(0), TW(0)};
+    if ((reg == 0 && !a.first) || reg == 1) {
+      const TW* pe = reg == 0 ? pend_prv + jp * rn + eb : pend_cur + jpp * rn + eb;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (lane + 64 * h < m) pin[h] = pe[lane + 64 * h];
+    }
+    // the next tile's loads fly over this tile's work
+    if (t + NW < ntiles) {
+      load_rhs_row(xs[0], a, t + NW, tA, j, jp, rsrc[0]);
+      load_rhs_row(xs[1], a, t + NW, tA, j, jp, rsrc[1]);
+    }
+
+    // where this lane's column goes: pivot rows -> block j, reduced rows -> block jp
+    const int kc = c0 + cl;
+    const bool colok = kc < width;
+    TW* const blk = reg == 0 ? a.D : reg == 1 ? a.E : a.B;
+    const size_t bsz = reg == 2 ? bn : nn;
+    const int kcs = colok ? kc : 0;
+    TW* const dtop = blk + size_t(j) * bsz + size_t(kcs) * n;
+    TW* const drest = blk + size_t(jp) * bsz + size_t(kcs) * n;
+
+    // U12 = L11^-1 X_top (L11^-1 is lower triangular: blocks kb <= rt); the
+    // independent accumulators of the row tiles alternate on the matrix core
+    f4_t u[NB];
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) u[rt] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+      const f4_t bv = *reinterpret_cast<const f4_t*>(xt + cl * XLD + kb * 16 + 4 * g);
+      f4_t av[NB];
+#pragma unroll
+      for (int rt = kb; rt < NB; ++rt) av[rt] = *reinterpret_cast<const f4_t*>(Ls + (rt * 16 + cl) * LD + kb * 16 + 4 * g);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int rt = kb; rt < NB; ++rt) u[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], bv[i], u[rt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < NB; ++rt) {
+      // every read of X_top by this lane precedes the write (same lane, same slots)
+      *reinterpret_cast<f4_t*>(xt + cl * XLD + rt * 16 + 4 * g) = u[rt];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = rt * 16 + 4 * g + i;
+        if (colok && q < n) dtop[q] = u[rt][i];
+      }
+    }
+    if (reg == 0 && !a.first) {   // pending C_jp into the border slots
+      TW* pe = pend_prv + jp * rn + eb;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int e = lane + 64 * h;
+        if (e < m) {
+          const int cc = e / nr, rb = e - cc * nr;
+          xt[cc * XLD + NT + n + rb] += pin[h];
+          pe[e] = TW(0);
+        }
+      }
+    }
+    // X_rest' = X_rest - [L21; C_j U^-1] U12 (the minus sign is in Ls)
+    constexpr int NRT = NR / 16;
+    f4_t acc[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) acc[rt] = *reinterpret_cast<const f4_t*>(xt + cl * XLD + NT + rt * 16 + 4 * g);
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+      const f4_t bv = *reinterpret_cast<const f4_t*>(xt + cl * XLD + kb * 16 + 4 * g);
+      f4_t av[NRT];
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+        av[rt] = *reinterpret_cast<const f4_t*>(Ls + (NT + rt * 16 + cl) * LD + kb * 16 + 4 * g);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], bv[i], acc[rt], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = rt * 16 + 4 * g + i;   // rest position: reduced row q (< n), else border row q - n
+        if (colok && q < n) drest[q] = acc[rt][i];
+        // border rows back to their slots for the coalesced pass below
+        if (q >= n && q < n + nr) xt[cl * XLD + NT + q] = acc[rt][i];
+      }
+    // border rows: C_jp' (region 0), pending C_jpp += (region 1), F partial (region 2)
+    TW* const bdst = reg == 0 ? a.C + jp * rn + eb : reg == 1 ? pend_cur + jpp * rn + eb
+                                                                : a.Fpart + size_t(j) * nr * nx + eb;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = lane + 64 * h;
+      if (e < m) {
+        const int cc = e / nr, rb = e - cc * nr;
+        const TW v = xt[cc * XLD + NT + n + rb];
+        bdst[e] = reg == 1 ? pin[h] + v : v;
+      }
+    }
+  }
+}
+
+template <typename TS, int NB, int RESTMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) pair_lu_mfma_kernel(FactorArgs<TS, float> a) {
+  pair_lu_mfma_body<TS, NB, RESTMAX, NW>(a, blockIdx.x);
+}
+
+template <typename TS, int NB, int RESTMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> t) {
+  const int e = t.entry[blockIdx.x];
+  pair_lu_mfma_body<TS, NB, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
+// one wave per pair: the LDS (multipliers and one staged tile, 31 KB) then
+// admits five pairs per CU
+constexpr int kMfmaWaves = 1;
+
+}  // namespace
+
+bool pair_lu_mfma_fits(int n, int nr, int nx) { return n >= 1 && n <= 48 && nr <= 8 && n + nr <= 64 && nx <= 16; }
+
+template <typename TS>
+bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err) {
+  if (!pair_lu_mfma_fits(a.n, a.nr, a.nx)) return false;
+  hipLaunchKernelGGL((pair_lu_mfma_kernel<TS, 3, 64, kMfmaWaves>), dim3(a.npairs), dim3(64 * kMfmaWaves), 0, st,
+                     a);
+  *err = hipGetLastError();
+  return true;
+}
+
+template <typename TS>
+bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
+                                 hipStream_t st, hipError_t* err) {
+  if (!pair_lu_mfma_fits(n, nr, nx)) return false;
+  hipLaunchKernelGGL((pair_lu_mfma_batched<TS, 3, 64, kMfmaWaves>), dim3(nblocks), dim3(64 * kMfmaWaves), 0, st,
+                     t);
+  *err = hipGetLastError();
+  return true;
+}
+
+template bool launch_pair_lu_mfma<float>(FactorArgs<float, float> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma<double>(FactorArgs<double, float> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma_batched<float>(BatchTable<FactorArgs<float, float>> const&, int, int, int, int,
+                                                 hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma_batched<double>(BatchTable<FactorArgs<double, float>> const&, int, int, int,
+                                                  int, hipStream_t, hipError_t*);
+
+}  // namespace bcr

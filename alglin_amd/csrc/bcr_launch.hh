@@ -27,6 +27,11 @@ bool launch_pair_lu_tiled(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_
 bool pair_lu_cols_fits(int n, int nr, int nx);
 template <typename TS, typename TW>
 bool launch_pair_lu_cols(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
+// FP32 factors, n <= 48, n + nr <= 64, nx <= 16: one wave factors T, the
+// right-hand columns are two GEMMs on the matrix cores; false otherwise
+bool pair_lu_mfma_fits(int n, int nr, int nx);
+template <typename TS>
+bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err);
 
 template <typename TW, typename TX>
 hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);

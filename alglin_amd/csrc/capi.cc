@@ -690,6 +690,14 @@ int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows
 
 void* alglin_batch_stream(alglin_batch_t h) { return h ? static_cast<void*>(h->s->stream()) : nullptr; }
 
+int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant) {
+  return bguard(h, [&] {
+    if (variant != 0 && variant != 4 && variant != 5) throw std::invalid_argument("pair kernel variant: 0, 4 or 5");
+    h->s->set_pair_variant(variant);
+    return 0;
+  });
+}
+
 int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods) {
   return bguard(h, [&] {
     h->s->set_methods(std::vector<int32_t>(methods, methods + nsys));

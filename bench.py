@@ -226,6 +226,8 @@ def bench_batched(args):
     ref.generate(seed=args.seed, shift=4.0, sing_k=sing, eps=eps)
     mix = Batch(mixed=True)
     mix.plan(dims, operator=False)
+    if args.pair_kernel in (4, 5):
+        mix.set_pair_kernel(args.pair_kernel)
     mix.share_operator(ref)
     L = ref.size("x")
     if args.calibrate_copy:   # a kernel of known traffic (1 GiB read + 1 GiB written) for counter scaling
@@ -293,7 +295,7 @@ def bench_batched(args):
     tf = mix.time_factorize()
     cls_names = {0: "pair_lu_fast_batched (n <= 4)", 1: "pair_lu_fast_batched (n <= 8)",
                  2: "pair_lu_fast_batched (n <= 16)", 3: "pair_lu_split_batched (16 < n <= 32)",
-                 4: "pair_lu_cols_batched (32 < n <= 64)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
+                 4: "pair_lu_mfma_batched (32 < n <= 48, FP32; pair_lu_cols_batched with --pair-kernel 4)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
     per_cls = {}
     for lev, cls, nb, ms, by, fl in tf:
         c = per_cls.setdefault(int(cls), [0, 0.0, 0.0, 0.0])
@@ -387,7 +389,8 @@ def main():
                     help="systems timed one at a time through the plain FP64 BorderedCR (batched workloads)")
     ap.add_argument("--pair-kernel", type=int, default=0,
                     help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n, "
-                         "4 column-split waves for every n <= 64")
+                         "4 column-split waves for every n <= 64, 5 matrix-core right-hand update (FP32 factors, "
+                         "n <= 48); batched workloads: the kernel of the 32 < n <= 64 class (0, 4 or 5)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()

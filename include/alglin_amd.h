@@ -116,7 +116,9 @@ int alglin_bcr_fast_path(alglin_bcr_t h);
    column-split-waves kernel up to n = 64 and the register-tiled kernel
    beyond; 2 the two-role (panel wave + columns wave) kernel for
    16 < n <= 32; 3 the register-tiled kernel for every n; 4 the
-   column-split-waves kernel for every n <= 64 */
+   column-split-waves kernel for every n <= 64; 5 the matrix-core
+   right-hand update for FP32 factors with n <= 48, n + nr <= 64, nx <= 16
+   (FP64 factors and other shapes: the register-tiled kernel) */
 int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant);
 
 /* ---- precision (extension of the reference surface) -------------------------- */
@@ -202,6 +204,13 @@ void* alglin_batch_stream(alglin_batch_t h);
    (may exceed max_rows), -1 on error.  No counterpart in the reference: the
    measurement hook of the batched extension. */
 int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows);
+
+/* Kernel of the 32 < n <= 64 LU pair class: 0 default (= 5 where it applies,
+   else 4), 4 column-split waves (every column updated per step), 5
+   matrix-core right-hand update (FP32 factors, n <= 48, nr <= 8,
+   n + nr <= 64, nx <= 16; other shapes take the column-split kernel).
+   Drops captured graphs.  No counterpart in the reference (a tuning hook). */
+int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant);
 
 /* per-system pair elimination method for alglin_batch_solve (0 LU, 1 QR, 2 QRP) */
 int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods);

@@ -74,12 +74,16 @@ def test_batch_fp64_matches_oracle_per_system(oracle_mod):
         assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 0.1 * res + 1e-14, (s, res)
 
 
-def test_batch_mixed_reaches_tolerance_and_graph_is_exact():
+@pytest.mark.parametrize("variant", [0, 4, 5])
+def test_batch_mixed_reaches_tolerance_and_graph_is_exact(variant):
+    """Mixed batch to 1e-12 with each kernel of the 32 < n <= 48 LU class
+    (default, column-split waves, matrix-core right-hand update)."""
     dims = mixed_dims()
     dims[:, 6] = 0                                  # LU everywhere
-    print(f"case batch mixed nsys={len(dims)}", flush=True)
+    print(f"case batch mixed nsys={len(dims)} pair kernel {variant}", flush=True)
     bt = Batch(mixed=True)
     bt.plan(dims)
+    bt.set_pair_kernel(variant)
     bt.generate(seed=11, shift=4.0)
     b = random_rhs(bt, 2)
     B = DeviceArray.from_host(b)

@@ -239,9 +239,12 @@ def test_device_resident_path():
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_pair_kernel_variants_agree(oracle_mod, shape, dtype):
     """Every LU pair kernel (default, two-role, register-tiled, column-split
-    waves) gives the oracle's pivots and the same solution; the register-tiled
-    and column-split kernels do the same arithmetic in the same order, so
-    their factor panels agree bit for bit."""
+    waves, matrix-core right-hand update) gives the oracle's pivots and the
+    same solution; the register-tiled and column-split kernels do the same
+    arithmetic in the same order, so their factor panels agree bit for bit.
+    The matrix-core kernel (FP32 factors, n <= 48) factors T with the same
+    arithmetic (T bit for bit at the first level, where its inputs are the
+    same) and forms the right-hand blocks as GEMMs (D, E to FP32 rounding)."""
     N, n, qr, qx, nr, nx = shape
     print(f"case shape={shape} dtype={np.dtype(dtype).name}", flush=True)
     s = systems.gaussian_system(N, n, qr, qx, nr, nx, shift=3.0, seed=N + n)
@@ -256,7 +259,7 @@ def test_pair_kernel_variants_agree(oracle_mod, shape, dtype):
     tol = 1e-11 if dtype == np.float64 else 2e-4
     panels = np.arange(1, N)                              # blocks j that are a pair's pivot block
     fac = {}
-    for variant in (0, 2, 3, 4):
+    for variant in (0, 2, 3, 4, 5):
         g = gpu_solver(s, "LU", dtype)
         g.set_pair_kernel(variant)
         assert g.factorize(), g.last_error()
@@ -271,3 +274,9 @@ def test_pair_kernel_variants_agree(oracle_mod, shape, dtype):
                             E=f["E"].reshape(N, -1)[panels])
     for k in ("T", "D", "E"):
         assert np.array_equal(fac[3][k], fac[4][k]), k
+    if dtype == np.float32 and n <= 48 and n + nr <= 64 and nx <= 16:
+        first = np.arange(1, N, 2) - 1                    # level-0 pairs among `panels`
+        assert np.array_equal(fac[5]["T"][first], fac[4]["T"][first])
+        for k in ("D", "E"):
+            ref = fac[4][k]
+            assert np.abs(fac[5][k] - ref).max() <= 1e-4 * max(1.0, np.abs(ref).max()), k
