@@ -3,36 +3,39 @@
 // the border update cc:1107-1154), with the right-hand columns on the matrix
 // cores.  The augmented panel is the one of bcr_factor_generic.hip: R = 2n +
 // nr rows [row jp; row j; border], columns [T = col j | col jp | col jpp | xb].
-// One wave per pair (the LDS a pair needs, 31 KB, then admits five pairs per
-// CU; the steps are latency-bound, so pairs in flight are what counts).
+// Two launches per level, so that the latency-bound pivot steps run with
+// many pairs per CU (A needs 2 KB of LDS) and only B holds the LDS a GEMM
+// needs.
 //
-//  1. The T panel [E_jp; D_j; C_j] (R x n) in registers, rows over lanes
-//     (row lane + 64 s, s < 2), columns in 16-wide register vectors: LAPACK
-//     partial pivoting among the first 2n rows (first maximum), multiplier
-//     a * (1/pivot), fma updates -- the arithmetic of the other LU kernels,
-//     so T, the pivots and C_j U^-1 agree with them bit for bit.  The step
-//     loop is rolled within each 16-column block (column c read and written
-//     through a uniform register index), a few KB of code.  Rows do not
-//     move; their final positions are tracked.  T, perm and C_j U^-1 go to
-//     their blocks, and LDS gets
-//        Ls[0, NT)        L11^-1 (unit lower, forward substitution, one
-//                         column per lane, in place over L11)
-//        Ls[NT, NT + NR)  -[L21; C_j U^-1] (final positions n .. 2n + nr - 1)
-//  2. The right-hand part X = [col jp | col jpp | xb] in 16-column tiles,
-//     each inside one of the three column blocks.  A tile's sources are
-//     loaded one tile ahead, staged in LDS with the rows at their final
-//     positions (column-major, NT + NR slots per column), and
-//        U12     = L11^-1 X_top             -> pivot rows: D_j', E_j', B_j'
+//  A. pair_lu_t: the T panel [E_jp; D_j; C_j] (R x n), one wave per pair,
+//     in registers: rows over lanes (row lane + 64 s, s < 2), columns in
+//     16-wide register vectors; LAPACK partial pivoting among the first 2n
+//     rows (first maximum), multiplier a * (1/pivot), fma updates -- the
+//     arithmetic of the other LU kernels, so T, the pivots and C_j U^-1
+//     agree with them bit for bit.  The step loop is rolled within each
+//     16-column block (column c read and written through a uniform register
+//     index), a few KB of code.  Rows do not move; each row is stored at its
+//     final position (T, C_j U^-1, perm).
+//  B. pair_rhs_mfma: the right-hand part X = [col jp | col jpp | xb], two
+//     waves per pair.  The row positions are replayed from perm; LDS gets
+//     -L11 below its 16 x 16 diagonal blocks, the inverses of those unit
+//     lower blocks on them (forward substitution, one column per lane), and
+//     -[L21; C_j U^-1].  X goes in 16-column tiles, each inside one of the
+//     three column blocks, its sources loaded one tile ahead and staged in
+//     LDS with the rows at their final positions; then
+//        U12_kb  = inv(L_kk) (X_kb - sum_{m<kb} L_km U12_m)
+//                                           -> pivot rows: D_j', E_j', B_j'
 //        X_rest' = X_rest - [L21; C_j U^-1] U12
 //                                           -> reduced rows D_jp', E_jp', B_jp'
-//     run on v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation), the
-//     row tiles' independent accumulators alternating.  The sum over k runs
-//     as 16-wide blocks, lane group g = lane / 16 taking k = 16 kb + 4 g + i
-//     in the i-th MFMA of a block, so that every operand is one 16-byte LDS
-//     read per lane per four MFMAs.  The border rows (C_jp', pending C_jpp,
-//     F partial) leave through LDS in one coalesced pass per tile.
+//     on v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation).  The sum
+//     over k runs as 16-wide blocks, lane group g = lane / 16 taking k = 16 kb
+//     + 4 g + i in the i-th MFMA of a block: every LDS operand is one 16-byte
+//     read per lane per four MFMAs, and a block of U12 in the accumulator
+//     layout is the next products' B operand as it stands.  The border rows
+//     (C_jp', pending C_jpp, F partial) leave through LDS in one coalesced
+//     pass per tile.
 // The right-hand results differ from the rank-1 kernels in rounding only
-// (other summation order, the explicit inverse of L11).  A pivot whose
+// (other summation order, the inverted diagonal blocks).  A pivot whose
 // reciprocal overflows (a subnormal pivot) turns the finished T columns into
 // NaN here, where the rank-1 kernels leave them; the factorization is
 // unusable either way and the refinement reports it.
@@ -115,15 +118,138 @@ __device__ __forceinline__ void load_rhs_row(TS (&xs)[16], FactorArgs<TS, float>
   for (int cc = 0; cc < 16; ++cc) xs[cc] = src[(c0 + cc < width ? c0 + cc : width - 1) * stride];
 }
 
-template <typename TS, int NB, int RESTMAX, int NW>
-__device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a, int p) {
+// ---- A: the T panel, one wave per pair ------------------------------------------
+template <typename TS, int NB, int NRMAX>
+__device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, int p) {
   using TW = float;
   constexpr int NMAX = 16 * NB;
-  using G = MfmaGeom<NMAX, RESTMAX>;
+  __shared__ __align__(16) TW stage[(NRMAX + 1) * NMAX];   // border rows C_j, one zero row
+  if (p >= a.npairs) return;
+  const int lane = threadIdx.x & 63;
+  const int n = a.n, nr = a.nr, n2 = 2 * n;
+  const int R = n2 + nr;
+  int j, jp, jpp;
+  pair_index(p, a.L, a.N, j, jp, jpp);
+  const size_t nn = size_t(n) * n, rn = size_t(nr) * n;
+  const TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
+  const TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;
+
+  // the border rows C_j (+ pending C_j) first, coalesced, into LDS rows of
+  // NMAX (row nr stays zero for the other lanes)
+  for (int e = lane; e < (nr + 1) * NMAX; e += 64) stage[e] = TW(0);
+  for (int e = lane; e < nr * n; e += 64) {   // C_j is nr x n, column-major
+    const int col = e / nr, ri = e - col * nr;
+    TW x = TW(a.sC[j * rn + e]);
+    if (!a.first) x += pend_prv[j * rn + e] + pend_cur[j * rn + e];
+    stage[ri * NMAX + col] = x;
+  }
+  f16v V[2][NB];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int r = lane + 64 * s;
+    const bool top = r < n, mid = r >= n && r < n2, bord = r >= n2 && r < R;
+    // rows of block jp: E_jp; rows of block j: D_j
+    const TS* base = top ? a.sE + jp * nn + r : a.sD + j * nn + (mid ? r - n : 0);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int col = 16 * b + k, off = (col < n ? col : n - 1) * n;
+        V[s][b][k] = keep_if(TW(base[off]), (top || mid) && col < n);
+      }
+    if (R > 64 * s && n2 < 64 * (s + 1)) {   // this row set holds border rows
+      const TW* srow = stage + (bord ? r - n2 : nr) * NMAX;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int k = 0; k < 16; k += 4) {
+          const f4_t c4 = *reinterpret_cast<const f4_t*>(srow + 16 * b + k);
+          V[s][b][k] += c4.x;
+          V[s][b][k + 1] += c4.y;
+          V[s][b][k + 2] += c4.z;
+          V[s][b][k + 3] += c4.w;
+        }
+    }
+  }
+  int pos0 = lane, pos1 = lane + 64;   // positions of rows lane, lane + 64
+  int myperm = 0;
+  bool zero_pivot = false;
+  // Steps in blocks of 16: step c of block b reads and rewrites column c
+  // through a uniform register index (s_set_gpr_idx) and updates the columns
+  // of blocks b.. with the pivot row's entries, those of block b not after c
+  // getting a zero entry.  The loop stays rolled: a few KB of code instead of
+  // one copy per step.
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int cend = min(16 * b + 16, n);
+    for (int c = 16 * b; c < cend; ++c) {
+      const int cc = c - 16 * b;
+      const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
+      const bool c0 = lane < n2 && pos0 >= c;
+      const bool c1 = lane + 64 < n2 && pos1 >= c;
+      unsigned h0, l0, h1, l1;
+      wave::abs_key(x0, h0, l0);
+      wave::abs_key(x1, h1, l1);
+      // the better of this lane's two candidates (ties to the smaller position)
+      const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
+      bool zero;
+      const int who = wave::argmax_pos<false>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane, zero);
+      const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
+      const bool sp = __builtin_amdgcn_readlane(int(b1), who) != 0;
+      zero_pivot = zero_pivot || zero;
+      myperm = lane == c ? P + 1 : myperm;
+      // the pivot row moves to position c, the row at c to the pivot's old place
+      pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
+      pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+      const TW pv = wave::rl(sp ? x1 : x0, who);
+      const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
+      // rows below the pivot get the multiplier, the others keep column c
+      const bool u0 = lane < n2 ? pos0 > c : lane < R;
+      const bool u1 = lane + 64 < n2 ? pos1 > c : lane + 64 < R;
+      const TW m0 = x0 * rinv, m1 = x1 * rinv;
+      const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
+      V[0][b][cc] = u0 ? m0 : x0;
+      V[1][b][cc] = u1 ? m1 : x1;
+      // columns 1..15 of the block (column 0 is never after c)
+      update_after<1>(V[0][b], V[1][b], sp, who, cc, lv0, lv1);
+#pragma unroll
+      for (int bb = b + 1; bb < NB; ++bb)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          const TW u = wave::rl(sp ? V[1][bb][k] : V[0][bb][k], who);
+          V[0][bb][k] = fma(-lv0, u, V[0][bb][k]);
+          V[1][bb][k] = fma(-lv1, u, V[1][bb][k]);
+        }
+    }
+  }
+  if (zero_pivot && lane == 0) atomicOr(a.status, 1);
+  // T (2n x n, final row order) and C_j U^-1 to their blocks
+  TW* Tj = a.Tm + size_t(j) * tsize(n);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int r = lane + 64 * s;
+    const int q = r < n2 ? (s == 0 ? pos0 : pos1) : r;
+    if (r < R) {
+      TW* dst = q < n2 ? Tj + q : a.C + j * rn + (q - n2);
+      const int stride = q < n2 ? n2 : nr;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          if (16 * b + k < n) dst[size_t(16 * b + k) * stride] = V[s][b][k];
+    }
+  }
+  if (lane < n) a.perm[size_t(j) * n + lane] = myperm;
+}
+
+// ---- B: the right-hand part on the matrix cores ----------------------------------
+template <typename TS, int NB, int RESTMAX, int NW>
+__device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, int p) {
+  using TW = float;
+  using G = MfmaGeom<16 * NB, RESTMAX>;
   constexpr int NT = G::NT, NR = G::NR, LD = G::LD, XLD = G::XLD;
   __shared__ __align__(16) TW Ls[(NT + NR) * LD];
   __shared__ __align__(16) TW Xt[NW][16 * XLD];
-  __shared__ int posbuf[128];
   if (p >= a.npairs) return;   // uniform over the workgroup
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -134,175 +260,123 @@ __device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a
   const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
   TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
   TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;
+  const TW* Tj = a.Tm + size_t(j) * tsize(n);
 
-  if (w == 0) {
-    // ---- 1. the T panel, 16-column register vectors --------------------------------
-    // the border rows C_j (+ pending C_j) first, coalesced, into LDS rows of
-    // NMAX (row nr stays zero for the other lanes)
-    TW* stage = Xt[0];
-    for (int e = lane; e < (nr + 1) * NMAX; e += 64) stage[e] = TW(0);
-    for (int e = lane; e < nr * n; e += 64) {   // C_j is nr x n, column-major
-      const int col = e / nr, ri = e - col * nr;
-      TW x = TW(a.sC[j * rn + e]);
-      if (!a.first) x += pend_prv[j * rn + e] + pend_cur[j * rn + e];
-      stage[ri * NMAX + col] = x;
-    }
-    f16v V[2][NB];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int r = lane + 64 * s;
-      const bool top = r < n, mid = r >= n && r < n2, bord = r >= n2 && r < R;
-      // rows of block jp: E_jp; rows of block j: D_j
-      const TS* base = top ? a.sE + jp * nn + r : a.sD + j * nn + (mid ? r - n : 0);
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const int col = 16 * b + k, off = (col < n ? col : n - 1) * n;
-          V[s][b][k] = keep_if(TW(base[off]), (top || mid) && col < n);
-        }
-      if (R > 64 * s && n2 < 64 * (s + 1)) {   // this row set holds border rows
-        const TW* srow = stage + (bord ? r - n2 : nr) * NMAX;
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-          for (int k = 0; k < 16; k += 4) {
-            const f4_t c4 = *reinterpret_cast<const f4_t*>(srow + 16 * b + k);
-            V[s][b][k] += c4.x;
-            V[s][b][k + 1] += c4.y;
-            V[s][b][k + 2] += c4.z;
-            V[s][b][k + 3] += c4.w;
-          }
-      }
-    }
-    int pos0 = lane, pos1 = lane + 64;   // positions of rows lane, lane + 64
-    int myperm = 0;
-    bool zero_pivot = false;
-    // Steps in blocks of 16: step c of block b reads and rewrites column c
-    // through a uniform register index (s_set_gpr_idx) and updates the
-    // columns of blocks b.. with the pivot row's entries, those of block b
-    // not after c getting a zero entry.  The loop stays rolled: a few KB of
-    // code instead of one copy per step.
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int cend = min(16 * b + 16, n);
-      for (int c = 16 * b; c < cend; ++c) {
-        const int cc = c - 16 * b;
-        const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
-        const bool c0 = lane < n2 && pos0 >= c;
-        const bool c1 = lane + 64 < n2 && pos1 >= c;
-        unsigned h0, l0, h1, l1;
-        wave::abs_key(x0, h0, l0);
-        wave::abs_key(x1, h1, l1);
-        // the better of this lane's two candidates (ties to the smaller position)
-        const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
-        bool zero;
-        const int who = wave::argmax_pos<false>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane, zero);
-        const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
-        const bool sp = __builtin_amdgcn_readlane(int(b1), who) != 0;
-        zero_pivot = zero_pivot || zero;
-        myperm = lane == c ? P + 1 : myperm;
-        // the pivot row moves to position c, the row at c to the pivot's old place
-        pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
-        pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
-        const TW pv = wave::rl(sp ? x1 : x0, who);
-        const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
-        // rows below the pivot get the multiplier, the others keep column c
-        const bool u0 = lane < n2 ? pos0 > c : lane < R;
-        const bool u1 = lane + 64 < n2 ? pos1 > c : lane + 64 < R;
-        const TW m0 = x0 * rinv, m1 = x1 * rinv;
-        const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
-        V[0][b][cc] = u0 ? m0 : x0;
-        V[1][b][cc] = u1 ? m1 : x1;
-        // columns 1..15 of the block (column 0 is never after c)
-        update_after<1>(V[0][b], V[1][b], sp, who, cc, lv0, lv1);
-#pragma unroll
-        for (int bb = b + 1; bb < NB; ++bb)
-#pragma unroll
-          for (int k = 0; k < 16; ++k) {
-            const TW u = wave::rl(sp ? V[1][bb][k] : V[0][bb][k], who);
-            V[0][bb][k] = fma(-lv0, u, V[0][bb][k]);
-            V[1][bb][k] = fma(-lv1, u, V[1][bb][k]);
-          }
-      }
-    }
-    if (zero_pivot && lane == 0) atomicOr(a.status, 1);
-    posbuf[lane] = pos0;
-    posbuf[64 + lane] = pos1;
-
-    // T (2n x n, final row order) and C_j U^-1 to their blocks; L to LDS
-    TW* Tj = a.Tm + size_t(j) * tsize(n);
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int r = lane + 64 * s;
-      const int q = r < n2 ? (s == 0 ? pos0 : pos1) : r;
-      if (r < R) {
-        TW* dst = q < n2 ? Tj + q : a.C + j * rn + (q - n2);
-        const int stride = q < n2 ? n2 : nr;
-        TW* lrow = q < n ? Ls + q * LD : Ls + (NT + q - n) * LD;
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-          for (int k = 0; k < 16; ++k) {
-            const int col = 16 * b + k;
-            const TW v = V[s][b][k];
-            if (col < n) dst[size_t(col) * stride] = v;
-            // top rows: the strict lower part of L11; other rows: minus the multipliers
-            lrow[col] = q < n ? (col < q ? v : TW(0)) : (col < n ? -v : TW(0));
-          }
-      } else {
-        const int i = r - R;   // rest padding rows n + nr .. NR - 1 to zero
-        if (i < NR - (n + nr)) {
-          TW* lrow = Ls + (NT + n + nr + i) * LD;
-#pragma unroll
-          for (int k = 0; k < NMAX; ++k) lrow[k] = TW(0);
-        }
-      }
-    }
-    if (lane < n) a.perm[size_t(j) * n + lane] = myperm;
-
-    // L11^-1 in place, column `lane`: y_i = -sum_{k<i} L_ik y_k, y_lane = 1;
-    // rows beyond n are identity rows (their slots hold zeros)
-    TW y[NMAX];
-#pragma unroll
-    for (int i = 0; i < NMAX; ++i) y[i] = TW(0);
-#pragma unroll
-    for (int i = 0; i < NMAX; ++i) {
-      // four partial sums: four short dependency chains instead of one
-      TW t0 = TW(0), t1 = TW(0), t2 = TW(0), t3 = TW(0);
-      if (i < n) {
-        const TW* li = Ls + i * LD;
-#pragma unroll
-        for (int k0 = 0; k0 < i; k0 += 4) {
-          const f4_t l4 = *reinterpret_cast<const f4_t*>(li + k0);
-          t0 = fma(l4.x, y[k0], t0);
-          if (k0 + 1 < NMAX) t1 = fma(l4.y, y[k0 + 1 < NMAX ? k0 + 1 : 0], t1);
-          if (k0 + 2 < NMAX) t2 = fma(l4.z, y[k0 + 2 < NMAX ? k0 + 2 : 0], t2);
-          if (k0 + 3 < NMAX) t3 = fma(l4.w, y[k0 + 3 < NMAX ? k0 + 3 : 0], t3);
-        }
-      }
-      y[i] = lane == i ? TW(1) : -((t0 + t1) + (t2 + t3));
-      if (lane < NT) Ls[i * LD + lane] = y[i];
-    }
-  }
-  // T stored, L11^-1 and the rest multipliers in LDS, final positions known
-  __syncthreads();
-
-  // ---- 2. right-hand tiles on the matrix cores ------------------------------------
-  // Tiles lie inside one column block of the right-hand part (region 0: col
-  // jp, region 1: col jpp, both n wide; region 2: xb, nx wide), so the row
-  // sources and destinations are fixed per tile.
-  TW* xt = Xt[w];
-  const int g = lane >> 4, cl = lane & 15;
-  int slot[2];
+  // this lane's two panel rows as right-hand sources, and the first tile's
+  // loads, in flight while the multipliers are staged
   RowSrc rsrc[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const int r = lane + 64 * s;
     rsrc[s].cls = r < n ? 1 : r < n2 ? 2 : r < R ? 4 : 0;
     rsrc[s].off = r < n ? r : r < n2 ? r - n : r < R ? r - n2 : 0;
+  }
+  const int tA = (n + 15) / 16, tC = (nx + 15) / 16;
+  const int ntiles = 2 * tA + tC;
+  // the tile's raw source values, loaded one tile ahead: row lane + 64 s,
+  // column c0 + cc of the tile's region
+  TS xs[2][16];
+  if (w < ntiles) {
+    load_rhs_row(xs[0], a, w, tA, j, jp, rsrc[0]);
+    load_rhs_row(xs[1], a, w, tA, j, jp, rsrc[1]);
+  }
+
+  // Ls: top rows (final positions < n) hold -L11 below the diagonal blocks
+  // and the inverses of L11's unit lower 16 x 16 diagonal blocks on them
+  // (set below); rest rows hold -[L21; C_j U^-1].  Rows are final positions,
+  // so every read is a column of T (or of C_j U^-1).
+  // Lanes over the columns k of T (column-major: a lane reads along its
+  // column, and the writes of Ls rows are bank-conflict-free); all loads are
+  // clamped and issued before their first use (T blocks are 4-byte aligned
+  // only: n (2n + 1) floats each).  Top row q < n: L below the diagonal, negated
+  // outside the diagonal blocks; rest row n + i: -T row n + i; border rows
+  // and padding rows below.
+  {
+    const int k = lane, kk = k < n ? k : n - 1;
+    constexpr int PAIRS = (2 * NT + 2 * NW - 1) / (2 * NW);   // row pairs of T per wave
+    float2 tv[PAIRS];
+#pragma unroll
+    for (int h = 0; h < PAIRS; ++h) {
+      const int q0 = 2 * (w + NW * h), qc = q0 < n2 ? q0 : 0;
+      tv[h].x = Tj[qc + size_t(kk) * n2];
+      tv[h].y = Tj[qc + 1 + size_t(kk) * n2];
+    }
+#pragma unroll
+    for (int h = 0; h < PAIRS; ++h) {
+      const int q0 = 2 * (w + NW * h);
+      if (q0 < n2 && k < NT) {
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const int q = q0 + d;
+          const TW v = d == 0 ? tv[h].x : tv[h].y;
+          TW x;
+          int row;
+          if (q < n) {
+            row = q;
+            x = k < q ? ((k >> 4) == (q >> 4) ? v : -v) : TW(0);
+          } else {
+            row = NT + q - n;
+            x = -v;
+          }
+          Ls[row * LD + k] = keep_if(x, k < n);
+        }
+      }
+    }
+    // border rows (C_j U^-1, nr x n column-major) and the zero padding rows
+    if (w == 0 && k < NT) {
+      for (int ri = 0; ri < nr; ++ri)
+        Ls[(NT + n + ri) * LD + k] = keep_if(-a.C[j * rn + ri + size_t(kk) * nr], k < n);
+      for (int q = n; q < NT; ++q) Ls[q * LD + k] = TW(0);
+      for (int i = n + nr; i < NR; ++i) Ls[(NT + i) * LD + k] = TW(0);
+    }
+  }
+  // the final position of this lane's two rows: LAPACK's swap sequence replayed
+  int pos0 = lane, pos1 = lane + 64;
+  {
+    const int pv = lane < n ? a.perm[size_t(j) * n + lane] - 1 : 0;
+    for (int c = 0; c < n; ++c) {
+      const int P = __builtin_amdgcn_readlane(pv, c);
+      pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
+      pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+    }
+  }
+  __syncthreads();   // Ls loaded
+  if (w == 0 && lane < NT) {
+    // the inverse of diagonal block lane / 16 of L11 (unit lower), column
+    // lane % 16, by forward substitution in place; rows beyond n are zero,
+    // their inverse rows identity
+    const int b = lane >> 4, jj = lane & 15;
+    TW y[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) y[i] = TW(0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const TW* li = Ls + (16 * b + i) * LD + 16 * b;
+      TW t0 = TW(0), t1 = TW(0);
+#pragma unroll
+      for (int k0 = 0; k0 < i; k0 += 4) {
+        const f4_t l4 = *reinterpret_cast<const f4_t*>(li + k0);
+        t0 = fma(l4.x, y[k0], t0);
+        if (k0 + 1 < 16) t1 = fma(l4.y, y[k0 + 1 < 16 ? k0 + 1 : 0], t1);
+        if (k0 + 2 < 16) t0 = fma(l4.z, y[k0 + 2 < 16 ? k0 + 2 : 0], t0);
+        if (k0 + 3 < 16) t1 = fma(l4.w, y[k0 + 3 < 16 ? k0 + 3 : 0], t1);
+      }
+      y[i] = jj == i ? TW(1) : -(t0 + t1);
+    }
+    // every lane has read its block rows (reads precede these writes in this wave)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Ls[(16 * b + i) * LD + 16 * b + jj] = y[i];
+  }
+  __syncthreads();   // the diagonal inverses in place
+
+  TW* xt = Xt[w];
+  const int g = lane >> 4, cl = lane & 15;
+  int slot[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int r = lane + 64 * s;
     if (r < n2) {
-      const int q = posbuf[r];
+      const int q = s == 0 ? pos0 : pos1;
       slot[s] = q < n ? q : NT + q - n;
     } else if (r < R) {
       slot[s] = NT + r - n;
@@ -311,13 +385,6 @@ __device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a
       slot[s] = i < NT - n ? n + i : i - (NT - n) < NR - (n + nr) ? NT + n + nr + (i - (NT - n)) : NT + NR;
     }
   }
-  const int tA = (n + 15) / 16, tC = (nx + 15) / 16;
-  const int ntiles = 2 * tA + tC;
-  // the tile's raw source values, loaded one tile ahead: row lane + 64 s,
-  // column c0 + cc of the tile's region
-  TS xs[2][16];
-  load_rhs_row(xs[0], a, w, tA, j, jp, rsrc[0]);
-  load_rhs_row(xs[1], a, w, tA, j, jp, rsrc[1]);
   for (int t = w; t < ntiles; t += NW) {
     const int reg = t < tA ? 0 : t < 2 * tA ? 1 : 2;
     const int c0 = 16 * (reg == 0 ? t : reg == 1 ? t - tA : t - 2 * tA);
@@ -347,42 +414,6 @@ __device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a
       load_rhs_row(xs[0], a, t + NW, tA, j, jp, rsrc[0]);
       load_rhs_row(xs[1], a, t + NW, tA, j, jp, rsrc[1]);
     }
-
-    // where this lane's column goes: pivot rows -> block j, reduced rows -> block jp
-    const int kc = c0 + cl;
-    const bool colok = kc < width;
-    TW* const blk = reg == 0 ? a.D : reg == 1 ? a.E : a.B;
-    const size_t bsz = reg == 2 ? bn : nn;
-    const int kcs = colok ? kc : 0;
-    TW* const dtop = blk + size_t(j) * bsz + size_t(kcs) * n;
-    TW* const drest = blk + size_t(jp) * bsz + size_t(kcs) * n;
-
-    // U12 = L11^-1 X_top (L11^-1 is lower triangular: blocks kb <= rt); the
-    // independent accumulators of the row tiles alternate on the matrix core
-    f4_t u[NB];
-#pragma unroll
-    for (int rt = 0; rt < NB; ++rt) u[rt] = f4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < NB; ++kb) {
-      const f4_t bv = *reinterpret_cast<const f4_t*>(xt + cl * XLD + kb * 16 + 4 * g);
-      f4_t av[NB];
-#pragma unroll
-      for (int rt = kb; rt < NB; ++rt) av[rt] = *reinterpret_cast<const f4_t*>(Ls + (rt * 16 + cl) * LD + kb * 16 + 4 * g);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int rt = kb; rt < NB; ++rt) u[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], bv[i], u[rt], 0, 0, 0);
-    }
-#pragma unroll
-    for (int rt = 0; rt < NB; ++rt) {
-      // every read of X_top by this lane precedes the write (same lane, same slots)
-      *reinterpret_cast<f4_t*>(xt + cl * XLD + rt * 16 + 4 * g) = u[rt];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int q = rt * 16 + 4 * g + i;
-        if (colok && q < n) dtop[q] = u[rt][i];
-      }
-    }
     if (reg == 0 && !a.first) {   // pending C_jp into the border slots
       TW* pe = pend_prv + jp * rn + eb;
 #pragma unroll
@@ -395,14 +426,42 @@ __device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a
         }
       }
     }
-    // X_rest' = X_rest - [L21; C_j U^-1] U12 (the minus sign is in Ls)
+
+    // where this lane's column goes: pivot rows -> block j, reduced rows -> block jp
+    const int kc = c0 + cl;
+    const bool colok = kc < width;
+    TW* const blk = reg == 0 ? a.D : reg == 1 ? a.E : a.B;
+    const size_t bsz = reg == 2 ? bn : nn;
+    const int kcs = colok ? kc : 0;
+    TW* const dtop = blk + size_t(j) * bsz + size_t(kcs) * n;
+    TW* const drest = blk + size_t(jp) * bsz + size_t(kcs) * n;
+
+    // U12 = L11^-1 X_top by 16-row blocks: u_kb = inv(L_kk) (X_kb - sum_{m<kb}
+    // L_km u_m).  A block of U12 in the accumulator layout (rows 4g + i,
+    // column cl) is the B operand layout of the next products as it stands.
+    f4_t u[NB];
+#pragma unroll
+    for (int kb = 0; kb < NB; ++kb) {
+      f4_t acc = *reinterpret_cast<const f4_t*>(xt + cl * XLD + kb * 16 + 4 * g);
+#pragma unroll
+      for (int mb = 0; mb < kb; ++mb)   // -L_km is in Ls
+        acc = mfma4(*reinterpret_cast<const f4_t*>(Ls + (kb * 16 + cl) * LD + mb * 16 + 4 * g), u[mb], acc);
+      u[kb] = mfma4(*reinterpret_cast<const f4_t*>(Ls + (kb * 16 + cl) * LD + kb * 16 + 4 * g), acc,
+                    f4_t{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = kb * 16 + 4 * g + i;
+        if (colok && q < n) dtop[q] = u[kb][i];
+      }
+    }
+    // X_rest' = X_rest - [L21; C_j U^-1] U12 (the minus sign is in Ls); the
+    // row tiles' independent accumulators alternate on the matrix core
     constexpr int NRT = NR / 16;
     f4_t acc[NRT];
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) acc[rt] = *reinterpret_cast<const f4_t*>(xt + cl * XLD + NT + rt * 16 + 4 * g);
 #pragma unroll
     for (int kb = 0; kb < NB; ++kb) {
-      const f4_t bv = *reinterpret_cast<const f4_t*>(xt + cl * XLD + kb * 16 + 4 * g);
       f4_t av[NRT];
 #pragma unroll
       for (int rt = 0; rt < NRT; ++rt)
@@ -410,7 +469,8 @@ __device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int rt = 0; rt < NRT; ++rt) acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], bv[i], acc[rt], 0, 0, 0);
+        for (int rt = 0; rt < NRT; ++rt)
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], u[kb][i], acc[rt], 0, 0, 0);
     }
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
@@ -436,20 +496,27 @@ __device__ __forceinline__ void pair_lu_mfma_body(FactorArgs<TS, float> const& a
   }
 }
 
-template <typename TS, int NB, int RESTMAX, int NW>
-__global__ void __launch_bounds__(64 * NW) pair_lu_mfma_kernel(FactorArgs<TS, float> a) {
-  pair_lu_mfma_body<TS, NB, RESTMAX, NW>(a, blockIdx.x);
+template <typename TS, int NB, int NRMAX>
+__global__ void __launch_bounds__(64) pair_lu_t_kernel(FactorArgs<TS, float> a) {
+  pair_lu_t_body<TS, NB, NRMAX>(a, blockIdx.x);
 }
-
-template <typename TS, int NB, int RESTMAX, int NW>
-__global__ void __launch_bounds__(64 * NW) pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> t) {
+template <typename TS, int NB, int NRMAX>
+__global__ void __launch_bounds__(64) pair_lu_t_batched(BatchTable<FactorArgs<TS, float>> t) {
   const int e = t.entry[blockIdx.x];
-  pair_lu_mfma_body<TS, NB, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
+  pair_lu_t_body<TS, NB, NRMAX>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+template <typename TS, int NB, int RESTMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_kernel(FactorArgs<TS, float> a) {
+  pair_rhs_body<TS, NB, RESTMAX, NW>(a, blockIdx.x);
+}
+template <typename TS, int NB, int RESTMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_batched(BatchTable<FactorArgs<TS, float>> t) {
+  const int e = t.entry[blockIdx.x];
+  pair_rhs_body<TS, NB, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
-// one wave per pair: the LDS (multipliers and one staged tile, 31 KB) then
-// admits five pairs per CU
-constexpr int kMfmaWaves = 1;
+// waves per pair in the right-hand kernel
+constexpr int kRhsWaves = 2;
 
 }  // namespace
 
@@ -458,8 +525,8 @@ bool pair_lu_mfma_fits(int n, int nr, int nx) { return n >= 1 && n <= 48 && nr <
 template <typename TS>
 bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err) {
   if (!pair_lu_mfma_fits(a.n, a.nr, a.nx)) return false;
-  hipLaunchKernelGGL((pair_lu_mfma_kernel<TS, 3, 64, kMfmaWaves>), dim3(a.npairs), dim3(64 * kMfmaWaves), 0, st,
-                     a);
+  hipLaunchKernelGGL((pair_lu_t_kernel<TS, 3, 8>), dim3(a.npairs), dim3(64), 0, st, a);
+  hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 3, 64, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st, a);
   *err = hipGetLastError();
   return true;
 }
@@ -468,8 +535,8 @@ template <typename TS>
 bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err) {
   if (!pair_lu_mfma_fits(n, nr, nx)) return false;
-  hipLaunchKernelGGL((pair_lu_mfma_batched<TS, 3, 64, kMfmaWaves>), dim3(nblocks), dim3(64 * kMfmaWaves), 0, st,
-                     t);
+  hipLaunchKernelGGL((pair_lu_t_batched<TS, 3, 8>), dim3(nblocks), dim3(64), 0, st, t);
+  hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 3, 64, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st, t);
   *err = hipGetLastError();
   return true;
 }
