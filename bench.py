@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0         # MI355X HBM3E, AMD published
 FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector/matrix, AMD published
+FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 dense matrix, AMD published
 
 
 def pair_bytes(n, nr, nx, s=8):
@@ -295,7 +296,8 @@ def bench_batched(args):
     tf = mix.time_factorize()
     cls_names = {0: "pair_lu_fast_batched (n <= 4)", 1: "pair_lu_fast_batched (n <= 8)",
                  2: "pair_lu_fast_batched (n <= 16)", 3: "pair_lu_split_batched (16 < n <= 32)",
-                 4: "pair_lu_mfma_batched (32 < n <= 48, FP32; pair_lu_cols_batched with --pair-kernel 4)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
+                 4: "pair_lu_t_batched + pair_rhs_mfma_batched (32 < n <= 48, FP32 factors; "
+                    "pair_lu_cols_batched with --pair-kernel 4)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
     per_cls = {}
     for lev, cls, nb, ms, by, fl in tf:
         c = per_cls.setdefault(int(cls), [0, 0.0, 0.0, 0.0])
@@ -349,6 +351,7 @@ def bench_batched(args):
             "avg_launch_ms": ms_d / nl,
             "launches": nl,
             "flop_rate_tflops": fl_d / (ms_d * 1e-3) / 1e12,
+            "flop_frac_fp32_peak": fl_d / (ms_d * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
             "factorize_ms_by_class": {cls_names.get(k, str(k)): v[1] for k, v in sorted(per_cls.items())},
             "whole_solve_algorithmic_gbs": nbytes / tmix / 1e9,
         },
