@@ -79,6 +79,9 @@ constexpr int kSweepNmax[4] = {4, 8, 16, 32};
 template <typename TA, typename TW>
 BatchEngine<TA, TW>::BatchEngine() {
   hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&join_, hipEventDisableTiming), "hipEventCreate");
   int dev = 0, lds = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
@@ -87,7 +90,22 @@ BatchEngine<TA, TW>::BatchEngine() {
 
 template <typename TA, typename TW>
 BatchEngine<TA, TW>::~BatchEngine() {
+  if (fork_) (void)hipEventDestroy(fork_);
+  if (join_) (void)hipEventDestroy(join_);
+  if (st2_) (void)hipStreamDestroy(st2_);
   if (st_) (void)hipStreamDestroy(st_);
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::fork() {
+  hip_check(hipEventRecord(fork_, st_), "fork");
+  hip_check(hipStreamWaitEvent(st2_, fork_, 0), "fork");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::join() {
+  hip_check(hipEventRecord(join_, st2_), "join");
+  hip_check(hipStreamWaitEvent(st_, join_, 0), "join");
 }
 
 template <typename TA, typename TW>
@@ -171,10 +189,60 @@ void BatchEngine<TA, TW>::invalidate() {
 
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::set_active(std::vector<char> const& mask) {
+  widen();
   active_.clear();
   for (int s = 0; s < nsys(); ++s)
     if (s < int(mask.size()) && mask[s]) active_.push_back(s);
   invalidate();
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::swap_stash() {
+  stash_.active.swap(active_);
+  stash_.fac.swap(fac_);
+  std::swap(stash_.prep, prep_);
+  std::swap(stash_.fred, fred_);
+  std::swap(stash_.asm_, asm_);
+  std::swap(stash_.lu, lu_);
+  std::swap(stash_.stptr, stptr_);
+  std::swap(stash_.fblob.p, fblob_.p);
+  std::swap(stash_.fblob.count, fblob_.count);
+  std::swap(stash_.tables_ok, tables_ok_);
+  stash_.solve.swap(solve_cache_);
+  stash_.mv.swap(mv_cache_);
+  stash_.norm.swap(norm_cache_);
+  stash_.add.swap(add_cache_);
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::narrow(std::vector<char> const& mask) {
+  widen();
+  swap_stash();   // the current tables aside; the live ones are now empty
+  stash_.has = true;
+  active_.clear();
+  for (int s = 0; s < nsys(); ++s)
+    if (s < int(mask.size()) && mask[s]) active_.push_back(s);
+  tables_ok_ = false;
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::widen() {
+  if (!stash_.has) return;
+  swap_stash();   // the narrowed tables into the stash, freed below
+  stash_.has = false;
+  stash_.active.clear();
+  stash_.fac.clear();
+  stash_.prep = nullptr;
+  stash_.fred = nullptr;
+  stash_.asm_ = nullptr;
+  stash_.lu = nullptr;
+  stash_.stptr = nullptr;
+  stash_.fblob.release();
+  stash_.tables_ok = false;
+  stash_.solve.clear();
+  stash_.mv.clear();
+  stash_.norm.clear();
+  stash_.add.clear();
 }
 
 template <typename TA, typename TW>
@@ -367,35 +435,40 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
   hip_check(launch_convert<int32_t, int32_t>(nullptr, skip_.p, nsys(), st_), "skip flags");
   hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
   hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
+  // the n > 32 class on st2_ unless every launch is timed (then one stream)
+  const bool conc = ev == nullptr;
+  if (conc) fork();
   size_t li = 0;
   for (int k = 0; k < levels_; ++k)
     for (Launch const& l : fac_[k]) {
-      if (ev) hip_check(hipEventRecord((*ev)[2 * li], st_), "event");
+      hipStream_t s = class_stream(l.cls, conc);
+      if (ev) hip_check(hipEventRecord((*ev)[2 * li], s), "event");
       hipError_t e = hipSuccess;
       auto run = [&](auto const* args) {
         using A = std::remove_cv_t<std::remove_pointer_t<decltype(args)>>;
         BatchTable<A> t{args, l.entry, l.first};
         bool ok = true;
-        if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
-        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, st_, &e);
-        else if (l.cls == 4 && try_mfma(pair_variant_, t, l, st_, &e))
+        if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
+        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, s, &e);
+        else if (l.cls == 4 && try_mfma(pair_variant_, t, l, s, &e))
           ok = true;
         else if (l.cls == 4 && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
-          ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
+          ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
-          ok = launch_pair_lu_tiled_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st_, &e);
+          ok = launch_pair_lu_tiled_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
         else {
           const size_t bytes = size_t(2 * l.nmax + l.nrmax) * (3 * l.nmax + l.nxmax) * sizeof(TW);
-          e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, st_);
+          e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, s);
         }
         if (!ok) throw HipError("batch factorize: no kernel for a class");
       };
       if (k == 0) run(static_cast<const FactorArgs<TA, TW>*>(l.args));
       else run(static_cast<const FactorArgs<TW, TW>*>(l.args));
       hip_check(e, "batch pair elimination");
-      if (ev) hip_check(hipEventRecord((*ev)[2 * li + 1], st_), "event");
+      if (ev) hip_check(hipEventRecord((*ev)[2 * li + 1], s), "event");
       ++li;
     }
+  if (conc) join();
   hip_check(launch_convert<TA, double>(pF_, Feff_.p, tot_[kbF], st_), "F");
   hip_check(launch_reduce_batched<TW, double>(fred_, ns, st_), "reduce F");
   hip_check(launch_assemble_last_batched<TW, TA>(asm_, ns, st_), "assemble last");
@@ -525,26 +598,32 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
   SolveTables& T = solve_tables(X);
   const int ns = int(active_.size());
   if (ns == 0) return;
+  // the n > 32 class's sweeps on st2_, concurrent with the other classes'
   auto sweep = [&](Launch const& l, bool fwd) {
     BatchTable<SolveArgs<TW, double>> t{static_cast<const SolveArgs<TW, double>*>(l.args), l.entry, l.first};
+    hipStream_t s = class_stream(l.cls, true);
     hipError_t e = hipSuccess;
     if (l.cls <= 3) {
-      if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, st_, &e))
+      if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, s, &e))
         throw HipError("batch solve: no sweep kernel");
     } else if (l.cls == 4) {
-      if (!launch_sweep_lu_stream_batched(fwd, t, l.nblocks, l.nmax, st_, &e))
+      if (!launch_sweep_lu_stream_batched(fwd, t, l.nblocks, l.nmax, s, &e))
         throw HipError("batch solve: no sweep kernel");
     } else {
-      e = launch_sweep_generic_batched(fwd, l.method, t, l.nblocks, l.nmax, l.nxmax, st_);
+      e = launch_sweep_generic_batched(fwd, l.method, t, l.nblocks, l.nmax, l.nxmax, s);
     }
     hip_check(e, fwd ? "batch forward" : "batch backward");
   };
+  fork();
   for (int k = 0; k < levels_; ++k)
     for (Launch const& l : T.fwd[k]) sweep(l, true);
+  join();
   hip_check(launch_reduce_batched<double, double>(T.xb, ns, st_), "reduce xb");
   hip_check(launch_last_solve_batched<double>(T.last, ns, Nrmax_, Ncmax_, lds_limit_, st_), "last solve");
+  fork();
   for (int k = levels_ - 1; k >= 0; --k)
     for (Launch const& l : T.fwd[k]) sweep(l, false);
+  join();
 }
 
 template <typename TA, typename TW>
@@ -829,8 +908,9 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
   ConvergeArgs ca{norms_.p, dstat_.p, dbest_.p, dstag_.p, eng.skip_flags(), drel_.p, dsweeps_.p, dopen_.p, tol, 0,
                   ns, giveup};
   int32_t open = 0;
+  bool narrowed = false;
   for (int it = 0;; ++it) {
-    if (graph) {
+    if (graph && !narrowed) {
       hip_check(hipGraphLaunch(g_.res, st), "graph residual");
     } else {
       eng.residual_async(X, B, R_.p);
@@ -843,7 +923,16 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
     hip_check(hipStreamSynchronize(st), "open count sync");
     rep.sweeps = std::max(rep.sweeps, it);
     if (open == 0 || it >= max_iter) break;
-    if (graph) {
+    if (!narrowed && int64_t(open) * 16 < ns) {
+      // few systems left: from here on launch only their pairs (every sweep
+      // over the whole batch costs its launches and empty workgroups)
+      hip_check(hipMemcpy(hd.data(), eng.skip_flags(), i4, hipMemcpyDeviceToHost), "done flags");
+      std::vector<char> mask(ns);
+      for (int s = 0; s < ns; ++s) mask[s] = hd[s] == 0;
+      eng.narrow(mask);
+      narrowed = true;
+    }
+    if (graph && !narrowed) {
       hip_check(hipGraphLaunch(g_.corr, st), "graph correction");
     } else {
       eng.solve_async(R_.p);
@@ -855,6 +944,7 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
   hip_check(hipMemcpyAsync(hd.data(), eng.skip_flags(), i4, hipMemcpyDeviceToHost, st), "done flags");
   hip_check(hipStreamSynchronize(st), "refinement results");
   for (int s = 0; s < ns; ++s) done[s] = hd[s] != 0;
+  if (narrowed) eng.widen();
 }
 
 template <typename E>

@@ -96,6 +96,12 @@ class BatchEngine {
   // per-system flags (device, nsys): the residual, norms, solve and add
   // launches skip a system whose flag is nonzero; factorize_async clears them
   int32_t* skip_flags() { return skip_.p; }
+  // The refinement's tail: narrow the active set to `mask` (launch tables
+  // for those systems only), keeping the current tables -- and so every
+  // graph captured over them -- aside; widen() brings them back unchanged.
+  // Nothing may be in flight on the engine's streams at either call.
+  void narrow(std::vector<char> const& mask);
+  void widen();
   // kernel of the 32 < n <= 64 LU class: 0 default (the matrix-core kernel
   // where it applies: FP32 factors, n <= 48; else column-split waves), 4
   // column-split waves, 5 matrix-core right-hand update
@@ -159,6 +165,14 @@ class BatchEngine {
   int maxN_ = 0, levels_ = 0, Ncmax_ = 0, Nrmax_ = 0;
   size_t lds_limit_ = 65536, factor_bytes_ = 0;
   hipStream_t st_ = nullptr;
+  // A second stream for the n > 32 class: the classes touch disjoint
+  // systems, so their level sequences run concurrently (forked from and
+  // joined to st_ by events, inside a captured graph too).
+  hipStream_t st2_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
+  void fork();
+  void join();
+  hipStream_t class_stream(int cls, bool concurrent) const { return concurrent && cls == 4 ? st2_ : st_; }
 
   TA const *pD_ = nullptr, *pE_ = nullptr, *pB_ = nullptr, *pC_ = nullptr, *pH_ = nullptr, *pCq_ = nullptr,
            *pF_ = nullptr;
@@ -180,6 +194,25 @@ class BatchEngine {
   std::map<std::vector<const void*>, MvTables> mv_cache_;
   std::map<std::vector<const void*>, NormTables> norm_cache_;
   std::map<std::vector<const void*>, AddTables> add_cache_;
+
+  // the tables set aside by narrow()
+  struct Stash {
+    bool has = false;
+    std::vector<int> active;
+    std::vector<std::vector<Launch>> fac;
+    const PrepArgs<TA, TW>* prep = nullptr;
+    const ReduceArgs<TW, double>* fred = nullptr;
+    const AssembleArgs<TW, TA>* asm_ = nullptr;
+    const LastSolveArgs<double>* lu = nullptr;
+    int32_t* const* stptr = nullptr;
+    DevBuf<char> fblob;
+    bool tables_ok = false;
+    std::map<const void*, SolveTables> solve;
+    std::map<std::vector<const void*>, MvTables> mv;
+    std::map<std::vector<const void*>, NormTables> norm;
+    std::map<std::vector<const void*>, AddTables> add;
+  } stash_;
+  void swap_stash();
 };
 
 // The batched solver a caller sees: FP64 cyclic reduction, or FP32 factors
