@@ -562,9 +562,10 @@ class Batch:
         return lib().alglin_batch_stream(self._h)
 
     def set_pair_kernel(self, variant: int):
-        """Kernel of the 32 < n <= 64 LU pair class: 0 default (the matrix-core
-        kernel where it applies, else column-split waves), 4 column-split waves,
-        5 matrix-core right-hand update (FP32 factors, n <= 48, nr <= 8)."""
+        """Kernels of the 16 < n <= 32 and 32 < n <= 64 LU pair classes: 0 default
+        (the matrix-core pair where it applies, else the rank-1 kernels), 4 the
+        rank-1 kernels (two-wave split, column-split), 5 the matrix-core pair
+        (FP32 factors, n <= 48, nr <= 8)."""
         self._check(lib().alglin_batch_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def time_factorize(self) -> np.ndarray:

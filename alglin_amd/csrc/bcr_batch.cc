@@ -13,9 +13,9 @@
 namespace bcr {
 namespace {
 
-// the matrix-core kernel of the 32 < n <= 48 LU class (FP32 factors only):
-// the default (variant 0) and variant 5; variant 4 keeps the column-split
-// kernel
+// the matrix-core pair of the 16 < n <= 32 and 32 < n <= 48 LU classes
+// (FP32 factors only): the default (variant 0) and variant 5; variant 4
+// keeps the rank-1 kernels (two-wave split, column-split)
 template <typename TS, typename TW, typename L>
 bool try_mfma(int variant, BatchTable<FactorArgs<TS, TW>> const& t, L const& l, hipStream_t st, hipError_t* e) {
   if constexpr (std::is_same_v<TW, float>) {
@@ -449,9 +449,9 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
         BatchTable<A> t{args, l.entry, l.first};
         bool ok = true;
         if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
-        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, s, &e);
-        else if (l.cls == 4 && try_mfma(pair_variant_, t, l, s, &e))
+        else if ((l.cls == 3 || l.cls == 4) && try_mfma(pair_variant_, t, l, s, &e))
           ok = true;
+        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, s, &e);
         else if (l.cls == 4 && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
           ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))

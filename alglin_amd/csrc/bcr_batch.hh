@@ -102,9 +102,10 @@ class BatchEngine {
   // Nothing may be in flight on the engine's streams at either call.
   void narrow(std::vector<char> const& mask);
   void widen();
-  // kernel of the 32 < n <= 64 LU class: 0 default (the matrix-core kernel
-  // where it applies: FP32 factors, n <= 48; else column-split waves), 4
-  // column-split waves, 5 matrix-core right-hand update
+  // kernels of the 16 < n <= 32 and 32 < n <= 64 LU classes: 0 default (the
+  // matrix-core pair where it applies: FP32 factors, n <= 48; else the
+  // rank-1 kernels), 4 the rank-1 kernels (two-wave split, column-split),
+  // 5 the matrix-core pair
   void set_pair_variant(int v) { pair_variant_ = v; }
 
   // one eager factorization with HIP events around every pair-elimination

@@ -9,7 +9,7 @@
 //
 //  A. pair_lu_t: the T panel [E_jp; D_j; C_j] (R x n), one wave per pair,
 //     in registers: rows over lanes (row lane + 64 s, s < 2), columns in
-//     16-wide register vectors; LAPACK partial pivoting among the first 2n
+//     two (n <= 32) or three 16-wide register vectors; LAPACK partial pivoting among the first 2n
 //     rows (first maximum), multiplier a * (1/pivot), fma updates -- the
 //     arithmetic of the other LU kernels, so T, the pivots and C_j U^-1
 //     agree with them bit for bit.  The step loop is rolled within each
@@ -518,15 +518,35 @@ __global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_batched(BatchTable<Fact
 // waves per pair in the right-hand kernel
 constexpr int kRhsWaves = 2;
 
+// shapes: 1: n <= 32, n + nr <= 48 (two 16-column blocks); 2: n <= 48,
+// n + nr <= 64 (three); both nr <= 8, nx <= 16; 0: not covered
+int mfma_variant(int n, int nr, int nx) {
+  if (n < 1 || nr > 8 || nx > 16) return 0;
+  if (n <= 32 && n + nr <= 48) return 1;
+  if (n <= 48 && n + nr <= 64) return 2;
+  return 0;
+}
+
 }  // namespace
 
-bool pair_lu_mfma_fits(int n, int nr, int nx) { return n >= 1 && n <= 48 && nr <= 8 && n + nr <= 64 && nx <= 16; }
+bool pair_lu_mfma_fits(int n, int nr, int nx) { return mfma_variant(n, nr, nx) != 0; }
 
 template <typename TS>
 bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err) {
-  if (!pair_lu_mfma_fits(a.n, a.nr, a.nx)) return false;
-  hipLaunchKernelGGL((pair_lu_t_kernel<TS, 3, 8>), dim3(a.npairs), dim3(64), 0, st, a);
-  hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 3, 64, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st, a);
+  switch (mfma_variant(a.n, a.nr, a.nx)) {
+    case 1:
+      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 2, 8>), dim3(a.npairs), dim3(64), 0, st, a);
+      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 2, 48, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
+                         a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 3, 8>), dim3(a.npairs), dim3(64), 0, st, a);
+      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 3, 64, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
+                         a);
+      break;
+    default:
+      return false;
+  }
   *err = hipGetLastError();
   return true;
 }
@@ -534,9 +554,20 @@ bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipErro
 template <typename TS>
 bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err) {
-  if (!pair_lu_mfma_fits(n, nr, nx)) return false;
-  hipLaunchKernelGGL((pair_lu_t_batched<TS, 3, 8>), dim3(nblocks), dim3(64), 0, st, t);
-  hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 3, 64, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st, t);
+  switch (mfma_variant(n, nr, nx)) {
+    case 1:
+      hipLaunchKernelGGL((pair_lu_t_batched<TS, 2, 8>), dim3(nblocks), dim3(64), 0, st, t);
+      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 2, 48, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
+                         t);
+      break;
+    case 2:
+      hipLaunchKernelGGL((pair_lu_t_batched<TS, 3, 8>), dim3(nblocks), dim3(64), 0, st, t);
+      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 3, 64, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
+                         t);
+      break;
+    default:
+      return false;
+  }
   *err = hipGetLastError();
   return true;
 }

@@ -171,6 +171,11 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
     m[c] = keep_if(vm, row && c < n);
     u[c] = keep_if(vu, row && h == 0 && c < n && c >= rr);
   }
+  // the reciprocal of this lane's diagonal of U, formed once per pair: the
+  // bottom-up recurrence then multiplies instead of dividing on its critical
+  // path (x_c = s_c * (1/u_cc) rather than s_c / u_cc)
+  const TW ud = Tj[rc + size_t(rc) * n2];
+  const TX dinv = row && h == 0 ? TX(1) / TX(ud) : TX(1);
   const TW* Bj = a.B + size_t(j) * n * nx;
   const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
   const size_t vin_row = size_t(h == 0 ? jp : jpp) * n + rr;
@@ -240,10 +245,9 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
       if (c < n) {
         TW uc = u[c];
         asm volatile("" : "+v"(uc));
-        const TX den = rr == c ? TX(uc) : TX(1);
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
-          const TX xc = gbcast<G>(s[q] / den, base, c);
+          const TX xc = gbcast<G>(s[q] * dinv, base, c);
           s[q] = (rr < c) ? s[q] - TX(uc) * xc : ((rr == c) ? xc : s[q]);
         }
       }

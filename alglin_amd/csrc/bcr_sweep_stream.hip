@@ -106,6 +106,9 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
   const bool row = lane < n;
   const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
   const int rc = row ? lane : 0;   // lanes past n read row 0 and never store
+  // the reciprocal of this lane's diagonal of U, once per pair: the bottom-up
+  // recurrence multiplies instead of dividing (x_c = s_c * (1/u_cc))
+  const TX dinv = row ? TX(1) / TX(Tj[rc + size_t(rc) * n2]) : TX(1);
   for (int q = 0; q < nrhs; ++q) {
     TX* x = a.x + size_t(q) * a.ldx;
     const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
@@ -140,8 +143,7 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
       for (int i = 0; i < 8; ++i) {
         const int c = c1 - i;
         if (c >= 0) {
-          const TX den = lane == c ? TX(u[i]) : TX(1);
-          const TX xc = TX(rlane(double(s / den), c));
+          const TX xc = TX(rlane(double(s * dinv), c));
           if (lane < c) s -= TX(u[i]) * xc;
           else if (lane == c) s = xc;
         }

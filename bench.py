@@ -295,7 +295,9 @@ def bench_batched(args):
     # batch's stream
     tf = mix.time_factorize()
     cls_names = {0: "pair_lu_fast_batched (n <= 4)", 1: "pair_lu_fast_batched (n <= 8)",
-                 2: "pair_lu_fast_batched (n <= 16)", 3: "pair_lu_split_batched (16 < n <= 32)",
+                 2: "pair_lu_fast_batched (n <= 16)",
+                 3: "pair_lu_t_batched + pair_rhs_mfma_batched (16 < n <= 32, FP32 factors; "
+                    "pair_lu_split_batched with --pair-kernel 4)",
                  4: "pair_lu_t_batched + pair_rhs_mfma_batched (32 < n <= 48, FP32 factors; "
                     "pair_lu_cols_batched with --pair-kernel 4)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
     per_cls = {}
@@ -393,7 +395,8 @@ def main():
     ap.add_argument("--pair-kernel", type=int, default=0,
                     help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n, "
                          "4 column-split waves for every n <= 64, 5 matrix-core right-hand update (FP32 factors, "
-                         "n <= 48); batched workloads: the kernel of the 32 < n <= 64 class (0, 4 or 5)")
+                         "n <= 48); batched workloads: the kernels of the 16 < n <= 64 classes (0 default, "
+                         "4 rank-1, 5 matrix-core)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
     args = ap.parse_args()

@@ -205,10 +205,11 @@ void* alglin_batch_stream(alglin_batch_t h);
    measurement hook of the batched extension. */
 int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows);
 
-/* Kernel of the 32 < n <= 64 LU pair class: 0 default (= 5 where it applies,
-   else 4), 4 column-split waves (every column updated per step), 5
-   matrix-core right-hand update (FP32 factors, n <= 48, nr <= 8,
-   n + nr <= 64, nx <= 16; other shapes take the column-split kernel).
+/* Kernels of the 16 < n <= 32 and 32 < n <= 64 LU pair classes: 0 default
+   (= 5 where it applies, else 4), 4 the rank-1 kernels (two-wave split for
+   n <= 32, column-split waves beyond: every column updated per step), 5 the
+   matrix-core pair (FP32 factors, n <= 48, nr <= 8, nx <= 16, n + nr <= 48
+   for n <= 32 and <= 64 beyond; other shapes take the rank-1 kernels).
    Drops captured graphs.  No counterpart in the reference (a tuning hook). */
 int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant);
 

@@ -76,8 +76,8 @@ def test_batch_fp64_matches_oracle_per_system(oracle_mod):
 
 @pytest.mark.parametrize("variant", [0, 4, 5])
 def test_batch_mixed_reaches_tolerance_and_graph_is_exact(variant):
-    """Mixed batch to 1e-12 with each kernel of the 32 < n <= 48 LU class
-    (default, column-split waves, matrix-core right-hand update)."""
+    """Mixed batch to 1e-12 with each choice of kernels for the 16 < n <= 48
+    LU classes (default, the rank-1 kernels, the matrix-core pair)."""
     dims = mixed_dims()
     dims[:, 6] = 0                                  # LU everywhere
     print(f"case batch mixed nsys={len(dims)} pair kernel {variant}", flush=True)
