@@ -91,6 +91,38 @@ __device__ __forceinline__ void update_after(f16v& v0, f16v& v1, bool sp, int wh
   }
 }
 
+// X_rest' = X_rest - [L21; C_j U^-1] U12 over the first NRT 16-row tiles of
+// the rest rows (the minus sign is in Ls); the tiles' independent
+// accumulators alternate on the matrix core.  Reduced rows go to drest,
+// border rows back to their staging slots.
+template <int NB, int NRT, int NT, int LD, int XLD>
+__device__ __forceinline__ void rest_tiles(const f4_t (&u)[NB], const float* Ls, float* xt, int cl, int g, int n,
+                                           int nr, bool colok, float* drest) {
+  f4_t acc[NRT];
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt) acc[rt] = *reinterpret_cast<const f4_t*>(xt + cl * XLD + NT + rt * 16 + 4 * g);
+#pragma unroll
+  for (int kb = 0; kb < NB; ++kb) {
+    f4_t av[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+      av[rt] = *reinterpret_cast<const f4_t*>(Ls + (NT + rt * 16 + cl) * LD + kb * 16 + 4 * g);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int rt = 0; rt < NRT; ++rt)
+        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], u[kb][i], acc[rt], 0, 0, 0);
+  }
+#pragma unroll
+  for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int q = rt * 16 + 4 * g + i;   // rest position: reduced row q (< n), else border row q - n
+      if (colok && q < n) drest[q] = acc[rt][i];
+      if (q >= n && q < n + nr) xt[cl * XLD + NT + q] = acc[rt][i];
+    }
+}
+
 // A panel row of the right-hand part: its class (1: a row of block jp, 2: a
 // row of block j, 4: a border row, 0: none) and its row in that block.
 struct RowSrc {
@@ -454,33 +486,13 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
         if (colok && q < n) dtop[q] = u[kb][i];
       }
     }
-    // X_rest' = X_rest - [L21; C_j U^-1] U12 (the minus sign is in Ls); the
-    // row tiles' independent accumulators alternate on the matrix core
-    constexpr int NRT = NR / 16;
-    f4_t acc[NRT];
-#pragma unroll
-    for (int rt = 0; rt < NRT; ++rt) acc[rt] = *reinterpret_cast<const f4_t*>(xt + cl * XLD + NT + rt * 16 + 4 * g);
-#pragma unroll
-    for (int kb = 0; kb < NB; ++kb) {
-      f4_t av[NRT];
-#pragma unroll
-      for (int rt = 0; rt < NRT; ++rt)
-        av[rt] = *reinterpret_cast<const f4_t*>(Ls + (NT + rt * 16 + cl) * LD + kb * 16 + 4 * g);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int rt = 0; rt < NRT; ++rt)
-          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], u[kb][i], acc[rt], 0, 0, 0);
-    }
-#pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int q = rt * 16 + 4 * g + i;   // rest position: reduced row q (< n), else border row q - n
-        if (colok && q < n) drest[q] = acc[rt][i];
-        // border rows back to their slots for the coalesced pass below
-        if (q >= n && q < n + nr) xt[cl * XLD + NT + q] = acc[rt][i];
-      }
+    // X_rest' = X_rest - [L21; C_j U^-1] U12 (the minus sign is in Ls), over
+    // the 16-row tiles the n + nr rest rows occupy (one fewer than NR / 16
+    // when n + nr <= NR - 16)
+    if (n + nr <= NR - 16)
+      rest_tiles<NB, NR / 16 - 1, NT, LD, XLD>(u, Ls, xt, cl, g, n, nr, colok, drest);
+    else
+      rest_tiles<NB, NR / 16, NT, LD, XLD>(u, Ls, xt, cl, g, n, nr, colok, drest);
     // border rows: C_jp' (region 0), pending C_jpp += (region 1), F partial (region 2)
     TW* const bdst = reg == 0 ? a.C + jp * rn + eb : reg == 1 ? pend_cur + jpp * rn + eb
                                                                 : a.Fpart + size_t(j) * nr * nx + eb;
