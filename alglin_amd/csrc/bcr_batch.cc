@@ -1030,6 +1030,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
   eng.norms_async(B, R_.p, norms_.p);
   eng.status(status_.data());
   // 2. features (always: they are also the selector's training inputs), selector
+  std::vector<char> direct(ns, 0);   // flagged systems sent straight to the FP64 path
   dfeat_.resize(size_t(ns) * 12);
   eng.features_async(norms_.p, dfeat_.p);
   hip_check(hipMemcpy(feat_.data(), dfeat_.p, feat_.size() * sizeof(float), hipMemcpyDeviceToHost), "features");
@@ -1044,10 +1045,17 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       if (p[s] > threshold) {
         mask[s] = 1;
         chose_qrp_[s] = 1;
-        eng.set_method(s, kQRP);
         ++nq;
       }
-    if (nq > 0) {
+    if (fallback) {
+      // with the FP64 path behind it, a flagged system goes there at once:
+      // the systems FP32 LU cannot solve are nearly singular beyond FP32's
+      // reach, FP32 QRP does not rescue them either, and the FP32 sweeps
+      // they would stall in are the costly tail of the refinement
+      direct = mask;
+    } else if (nq > 0) {
+      for (int s = 0; s < ns; ++s)
+        if (mask[s]) eng.set_method(s, kQRP);
       eng.set_active(mask);
       eng.copy_active_async(B, X);
       eng.factorize_async();
@@ -1062,7 +1070,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
   // 3. FP64 refinement of every system with its factors; with the fallback
   // behind it, a system that would need more than six further sweeps at its
   // contraction rate is handed over at once instead of sweeping to max_iter
-  std::vector<char> done(ns, 0);
+  std::vector<char> done(direct.begin(), direct.end());
   refine(eng, B, X, tol, max_iter, done, rep, fallback ? 6 : 0);
   // 4. the FP64 fallback for the systems that did not get there: first the
   // plain FP64 path (LU elimination, FP64 refinement), then FP64 QRP for what
@@ -1078,7 +1086,13 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
     return k;
   };
   std::vector<char> all(ns, 1), bad(ns, 0);
-  if (fallback && failed(all, bad) > 0) {
+  int nbad = failed(all, bad);
+  for (int s = 0; s < ns; ++s)
+    if (direct[s] && !bad[s]) {
+      bad[s] = 1;
+      ++nbad;
+    }
+  if (fallback && nbad > 0) {
     if (!e64_) {
       e64_ = new BatchEngine<double, double>();
       e64_->plan(dims_);

@@ -239,10 +239,11 @@ class BatchSolver {
   // B, X: packed device vectors (kind x); X = A^{-1} B per system
   BatchReport solve(double const* B, double* X, double tol, int max_iter, bool use_graph);
   // mixed solver with per-system method choice: every system starts with FP32
-  // LU; after the first solve the selector (if given) sends the systems it
-  // scores above `threshold` to FP32 QRP; all are refined in FP64; with
-  // `fallback`, systems that still miss tol are solved in FP64: LU (the plain
-  // FP64 path) first, QRP for what that misses
+  // LU; after the first solve the selector (if given) scores each system.
+  // With `fallback`, the systems it scores above `threshold` go straight to
+  // the FP64 path (LU, then QRP for what LU misses) and the others are refined
+  // in FP64 and join that path if they still miss tol; without, the flagged
+  // systems are refactored with FP32 QRP and all are refined in FP64
   BatchReport solve_selected(double const* B, double* X, double tol, int max_iter, class Mlp* selector,
                              float threshold, bool fallback);
   // features of the last solve_selected (12 per system, host) and the

@@ -278,8 +278,12 @@ def bench_batched(args):
         mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=False)
         f_sel = int(mix.report()["failed"])
         out["without_fallback"] = dict(failed_fp32_lu_only=f_lu, failed_with_selector=f_sel)
-        out["selector"] = dict(sent_to_qrp=int(chosen["chose_qrp"].sum()),
-                               fp64_fallbacks=int(chosen["fell_back"].sum()),
+        # with the fallback on, flagged systems go straight to the FP64 path
+        # (LU, then QRP); the rest join it only if FP32 LU + refinement fails
+        fl = chosen["chose_qrp"]
+        out["selector"] = dict(flagged=int(fl.sum()),
+                               fp64_path=int(chosen["fell_back"].sum()),
+                               fp64_path_unflagged=int((chosen["fell_back"] & ~fl).sum()),
                                fp64_lu=int((chosen["fallback_stage"] == 1).sum()),
                                fp64_qrp=int((chosen["fallback_stage"] == 2).sum()),
                                singular_systems=int((sing >= 1).sum()))

@@ -241,11 +241,13 @@ int alglin_mlp_train(alglin_mlp_t h, float const* X, int32_t const* y, int32_t n
 /* probability of class 1 per sample (device output) */
 int alglin_mlp_predict(alglin_mlp_t h, float const* X, int32_t n, float* p1);
 
-/* mixed batch solve with per-system method choice: FP32 LU first; systems the
-   selector scores above threshold (selector may be null: none) are
-   refactored with FP32 QRP; FP64 refinement; with fallback=1 the systems that
-   still miss tol are solved in FP64: the plain FP64 path (LU) first, FP64
-   QRP for what that misses.  Return codes as alglin_batch_solve. */
+/* mixed batch solve with per-system method choice: FP32 LU first; the
+   selector (may be null: none) scores every system from that first solve.
+   With fallback=1, the systems it scores above threshold go straight to the
+   FP64 path (LU, then FP64 QRP for what LU misses), the others are refined
+   in FP64 and join that path if they still miss tol.  With fallback=0 the
+   flagged systems are refactored with FP32 QRP instead, and every system is
+   refined in FP64.  Return codes as alglin_batch_solve. */
 int alglin_batch_solve_selected(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter,
                                 alglin_mlp_t selector, float threshold, int fallback);
 /* of the last solve_selected: features (12 per system), systems sent to QRP,
