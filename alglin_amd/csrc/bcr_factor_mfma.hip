@@ -51,14 +51,14 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 // LDS geometry: NT top slots (pivot positions, NMAX a multiple of 16), NR
 // rest slots (reduced and border rows, n + nr <= RESTMAX)
-template <int NMAX, int RESTMAX>
+template <int NMAX, int RESTMAX, int RS>
 struct MfmaGeom {
   static constexpr int NT = NMAX;
   static constexpr int NR = (RESTMAX + 15) / 16 * 16;
   static constexpr int LD = NT + 4;        // row stride of Ls
   static constexpr int XLD = NT + NR + 4;  // column stride of a staged tile
   static_assert(NT % 16 == 0, "NMAX must be a multiple of 16");
-  static_assert(NT + NR <= 128, "padding slots come from the lanes beyond row R");
+  static_assert(NT + NR <= 64 * RS, "padding slots come from the lanes beyond row R");
 };
 
 __device__ __forceinline__ f4_t mfma4(f4_t a, f4_t b, f4_t c) {
@@ -79,15 +79,18 @@ __device__ __forceinline__ float rl_after(float v, int who, int cc) {
   return __int_as_float(x);
 }
 
-// columns K.. 15 of one 16-column block of both row sets, those not after
-// column cc of the block getting a zero pivot-row entry
-template <int K>
-__device__ __forceinline__ void update_after(f16v& v0, f16v& v1, bool sp, int who, int cc, float lv0, float lv1) {
+// columns K.. 15 of one 16-column block of the row sets, those not after
+// column cc of the block getting a zero pivot-row entry; the pivot row is in
+// row set 0 or 1 (a third row set holds border rows only)
+template <int K, int RS>
+__device__ __forceinline__ void update_after(f16v& v0, f16v& v1, f16v& v2, bool sp, int who, int cc, float lv0,
+                                             float lv1, float lv2) {
   if constexpr (K < 16) {
     const float u = rl_after<K>(sp ? v1[K] : v0[K], who, cc);
     v0[K] = fma(-lv0, u, v0[K]);
     v1[K] = fma(-lv1, u, v1[K]);
-    update_after<K + 1>(v0, v1, sp, who, cc, lv0, lv1);
+    if constexpr (RS > 2) v2[K] = fma(-lv2, u, v2[K]);
+    update_after<K + 1, RS>(v0, v1, v2, sp, who, cc, lv0, lv1, lv2);
   }
 }
 
@@ -151,10 +154,12 @@ __device__ __forceinline__ void load_rhs_row(TS (&xs)[16], FactorArgs<TS, float>
 }
 
 // ---- A: the T panel, one wave per pair ------------------------------------------
-template <typename TS, int NB, int NRMAX>
+template <typename TS, int NB, int RS, int NRMAX>
 __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, int p) {
   using TW = float;
   constexpr int NMAX = 16 * NB;
+  static_assert(RS == 2 || RS == 3, "two or three row sets");
+  static_assert(NMAX <= 64, "the 2n candidate rows lie in row sets 0 and 1");
   __shared__ __align__(16) TW stage[(NRMAX + 1) * NMAX];   // border rows C_j, one zero row
   if (p >= a.npairs) return;
   const int lane = threadIdx.x & 63;
@@ -175,9 +180,9 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
     if (!a.first) x += pend_prv[j * rn + e] + pend_cur[j * rn + e];
     stage[ri * NMAX + col] = x;
   }
-  f16v V[2][NB];
+  f16v V[RS][NB];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < RS; ++s) {
     const int r = lane + 64 * s;
     const bool top = r < n, mid = r >= n && r < n2, bord = r >= n2 && r < R;
     // rows of block jp: E_jp; rows of block j: D_j
@@ -217,6 +222,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
     for (int c = 16 * b; c < cend; ++c) {
       const int cc = c - 16 * b;
       const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
+      const TW x2 = RS > 2 ? V[RS - 1][b][cc] : TW(0);   // border rows (row set 2)
       const bool c0 = lane < n2 && pos0 >= c;
       const bool c1 = lane + 64 < n2 && pos1 >= c;
       unsigned h0, l0, h1, l1;
@@ -242,8 +248,15 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
       const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
       V[0][b][cc] = u0 ? m0 : x0;
       V[1][b][cc] = u1 ? m1 : x1;
+      TW lv2 = TW(0);
+      if constexpr (RS > 2) {   // border rows are never pivots: always updated
+        const bool u2 = lane + 128 < R;
+        const TW m2 = x2 * rinv;
+        lv2 = u2 ? m2 : TW(0);
+        V[RS - 1][b][cc] = u2 ? m2 : x2;
+      }
       // columns 1..15 of the block (column 0 is never after c)
-      update_after<1>(V[0][b], V[1][b], sp, who, cc, lv0, lv1);
+      update_after<1, RS>(V[0][b], V[1][b], V[RS - 1][b], sp, who, cc, lv0, lv1, lv2);
 #pragma unroll
       for (int bb = b + 1; bb < NB; ++bb)
 #pragma unroll
@@ -251,6 +264,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
           const TW u = wave::rl(sp ? V[1][bb][k] : V[0][bb][k], who);
           V[0][bb][k] = fma(-lv0, u, V[0][bb][k]);
           V[1][bb][k] = fma(-lv1, u, V[1][bb][k]);
+          if constexpr (RS > 2) V[RS - 1][bb][k] = fma(-lv2, u, V[RS - 1][bb][k]);
         }
     }
   }
@@ -258,7 +272,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
   // T (2n x n, final row order) and C_j U^-1 to their blocks
   TW* Tj = a.Tm + size_t(j) * tsize(n);
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < RS; ++s) {
     const int r = lane + 64 * s;
     const int q = r < n2 ? (s == 0 ? pos0 : pos1) : r;
     if (r < R) {
@@ -275,10 +289,10 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
 }
 
 // ---- B: the right-hand part on the matrix cores ----------------------------------
-template <typename TS, int NB, int RESTMAX, int NW>
+template <typename TS, int NB, int RS, int RESTMAX, int NW>
 __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, int p) {
   using TW = float;
-  using G = MfmaGeom<16 * NB, RESTMAX>;
+  using G = MfmaGeom<16 * NB, RESTMAX, RS>;
   constexpr int NT = G::NT, NR = G::NR, LD = G::LD, XLD = G::XLD;
   __shared__ __align__(16) TW Ls[(NT + NR) * LD];
   __shared__ __align__(16) TW Xt[NW][16 * XLD];
@@ -296,9 +310,9 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
 
   // this lane's two panel rows as right-hand sources, and the first tile's
   // loads, in flight while the multipliers are staged
-  RowSrc rsrc[2];
+  RowSrc rsrc[RS];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < RS; ++s) {
     const int r = lane + 64 * s;
     rsrc[s].cls = r < n ? 1 : r < n2 ? 2 : r < R ? 4 : 0;
     rsrc[s].off = r < n ? r : r < n2 ? r - n : r < R ? r - n2 : 0;
@@ -307,10 +321,10 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   const int ntiles = 2 * tA + tC;
   // the tile's raw source values, loaded one tile ahead: row lane + 64 s,
   // column c0 + cc of the tile's region
-  TS xs[2][16];
+  TS xs[RS][16];
   if (w < ntiles) {
-    load_rhs_row(xs[0], a, w, tA, j, jp, rsrc[0]);
-    load_rhs_row(xs[1], a, w, tA, j, jp, rsrc[1]);
+#pragma unroll
+    for (int s = 0; s < RS; ++s) load_rhs_row(xs[s], a, w, tA, j, jp, rsrc[s]);
   }
 
   // Ls: top rows (final positions < n) hold -L11 below the diagonal blocks
@@ -403,9 +417,9 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
 
   TW* xt = Xt[w];
   const int g = lane >> 4, cl = lane & 15;
-  int slot[2];
+  int slot[RS];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < RS; ++s) {
     const int r = lane + 64 * s;
     if (r < n2) {
       const int q = s == 0 ? pos0 : pos1;
@@ -423,7 +437,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     const int width = reg == 2 ? nx : n;
     // stage the tile: column cc at xt[cc * XLD + slot], zeros outside the region's rows
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < RS; ++s) {
       const int cls = rsrc[s].cls;
       const bool valid = reg == 0 ? (cls & 5) != 0 : reg == 1 ? cls == 2 : (cls & 3) != 0;
 #pragma unroll
@@ -443,8 +457,8 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     }
     // the next tile's loads fly over this tile's work
     if (t + NW < ntiles) {
-      load_rhs_row(xs[0], a, t + NW, tA, j, jp, rsrc[0]);
-      load_rhs_row(xs[1], a, t + NW, tA, j, jp, rsrc[1]);
+#pragma unroll
+      for (int s = 0; s < RS; ++s) load_rhs_row(xs[s], a, t + NW, tA, j, jp, rsrc[s]);
     }
     if (reg == 0 && !a.first) {   // pending C_jp into the border slots
       TW* pe = pend_prv + jp * rn + eb;
@@ -508,34 +522,36 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   }
 }
 
-template <typename TS, int NB, int NRMAX>
+template <typename TS, int NB, int RS, int NRMAX>
 __global__ void __launch_bounds__(64) pair_lu_t_kernel(FactorArgs<TS, float> a) {
-  pair_lu_t_body<TS, NB, NRMAX>(a, blockIdx.x);
+  pair_lu_t_body<TS, NB, RS, NRMAX>(a, blockIdx.x);
 }
-template <typename TS, int NB, int NRMAX>
+template <typename TS, int NB, int RS, int NRMAX>
 __global__ void __launch_bounds__(64) pair_lu_t_batched(BatchTable<FactorArgs<TS, float>> t) {
   const int e = t.entry[blockIdx.x];
-  pair_lu_t_body<TS, NB, NRMAX>(t.args[e], int(blockIdx.x) - t.first[e]);
+  pair_lu_t_body<TS, NB, RS, NRMAX>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
-template <typename TS, int NB, int RESTMAX, int NW>
+template <typename TS, int NB, int RS, int RESTMAX, int NW>
 __global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_kernel(FactorArgs<TS, float> a) {
-  pair_rhs_body<TS, NB, RESTMAX, NW>(a, blockIdx.x);
+  pair_rhs_body<TS, NB, RS, RESTMAX, NW>(a, blockIdx.x);
 }
-template <typename TS, int NB, int RESTMAX, int NW>
+template <typename TS, int NB, int RS, int RESTMAX, int NW>
 __global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_batched(BatchTable<FactorArgs<TS, float>> t) {
   const int e = t.entry[blockIdx.x];
-  pair_rhs_body<TS, NB, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
+  pair_rhs_body<TS, NB, RS, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
 // waves per pair in the right-hand kernel
 constexpr int kRhsWaves = 2;
 
 // shapes: 1: n <= 32, n + nr <= 48 (two 16-column blocks); 2: n <= 48,
-// n + nr <= 64 (three); both nr <= 8, nx <= 16; 0: not covered
+// n + nr <= 64 (three); 3: n <= 64, n + nr <= 80 (four, and a third row set
+// for the border rows beyond row 128); all nr <= 8, nx <= 16; 0: not covered
 int mfma_variant(int n, int nr, int nx) {
   if (n < 1 || nr > 8 || nx > 16) return 0;
   if (n <= 32 && n + nr <= 48) return 1;
   if (n <= 48 && n + nr <= 64) return 2;
+  if (n <= 64 && n + nr <= 80) return 3;
   return 0;
 }
 
@@ -547,13 +563,18 @@ template <typename TS>
 bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err) {
   switch (mfma_variant(a.n, a.nr, a.nx)) {
     case 1:
-      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 2, 8>), dim3(a.npairs), dim3(64), 0, st, a);
-      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 2, 48, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
+      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 2, 2, 8>), dim3(a.npairs), dim3(64), 0, st, a);
+      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 2, 2, 48, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
                          a);
       break;
     case 2:
-      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 3, 8>), dim3(a.npairs), dim3(64), 0, st, a);
-      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 3, 64, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
+      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 3, 2, 8>), dim3(a.npairs), dim3(64), 0, st, a);
+      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 3, 2, 64, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
+                         a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 4, 3, 8>), dim3(a.npairs), dim3(64), 0, st, a);
+      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 4, 3, 80, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
                          a);
       break;
     default:
@@ -568,13 +589,18 @@ bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int
                                  hipStream_t st, hipError_t* err) {
   switch (mfma_variant(n, nr, nx)) {
     case 1:
-      hipLaunchKernelGGL((pair_lu_t_batched<TS, 2, 8>), dim3(nblocks), dim3(64), 0, st, t);
-      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 2, 48, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
+      hipLaunchKernelGGL((pair_lu_t_batched<TS, 2, 2, 8>), dim3(nblocks), dim3(64), 0, st, t);
+      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 2, 2, 48, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
                          t);
       break;
     case 2:
-      hipLaunchKernelGGL((pair_lu_t_batched<TS, 3, 8>), dim3(nblocks), dim3(64), 0, st, t);
-      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 3, 64, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
+      hipLaunchKernelGGL((pair_lu_t_batched<TS, 3, 2, 8>), dim3(nblocks), dim3(64), 0, st, t);
+      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 3, 2, 64, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
+                         t);
+      break;
+    case 3:
+      hipLaunchKernelGGL((pair_lu_t_batched<TS, 4, 3, 8>), dim3(nblocks), dim3(64), 0, st, t);
+      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 4, 3, 80, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
                          t);
       break;
     default:

@@ -398,8 +398,8 @@ def main():
                     help="systems timed one at a time through the plain FP64 BorderedCR (batched workloads)")
     ap.add_argument("--pair-kernel", type=int, default=0,
                     help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n, "
-                         "4 column-split waves for every n <= 64, 5 matrix-core right-hand update (FP32 factors, "
-                         "n <= 48); batched workloads: the kernels of the 16 < n <= 64 classes (0 default, "
+                         "4 column-split waves for every n <= 64, 5 the matrix-core pair (FP32 factors, "
+                         "n <= 64); batched workloads: the kernels of the 16 < n <= 64 classes (0 default, "
                          "4 rank-1, 5 matrix-core)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a kernel of known traffic for counter scaling)")
@@ -547,7 +547,11 @@ def main():
             "avg_launch_ms": pair_ms_avg_launch,
             "launches_per_step": launches,
             "flop_rate_tflops": flops_per_factorize / launches / (pair_ms_avg_launch * 1e-3) / 1e12,
-            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+            # against the matrix peak of the factor precision (FP32 factors in
+            # mixed mode: the matrix-core pair for 16 < n <= 64)
+            "flop_peak_tflops": FP64_PEAK_TFLOPS if args.mode == "fp64" else FP32_PEAK_TFLOPS,
+            "flop_frac": flops_per_factorize / launches / (pair_ms_avg_launch * 1e-3) / 1e12
+                         / (FP64_PEAK_TFLOPS if args.mode == "fp64" else FP32_PEAK_TFLOPS),
         },
         "device": dinfo,
     }

@@ -112,13 +112,15 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T, int32_t* perm, void* D, void
                            double* last_block, double* pending_sumsq);
 /* 1 when the last factorization used the register-resident LU kernel */
 int alglin_bcr_fast_path(alglin_bcr_t h);
-/* LU pair kernel: 0 (default) register kernels for n <= 32, the
-   column-split-waves kernel up to n = 64 and the register-tiled kernel
-   beyond; 2 the two-role (panel wave + columns wave) kernel for
+/* LU pair kernel: 0 (default) the matrix-core pair for FP32 factors with
+   16 < n <= 64 where it applies (nr <= 8, nx <= 16), else register kernels
+   for n <= 32, the column-split-waves kernel up to n = 64 and the
+   register-tiled kernel beyond; 2 the two-role (panel wave + columns wave) kernel for
    16 < n <= 32; 3 the register-tiled kernel for every n; 4 the
-   column-split-waves kernel for every n <= 64; 5 the matrix-core
-   right-hand update for FP32 factors with n <= 48, n + nr <= 64, nx <= 16
-   (FP64 factors and other shapes: the register-tiled kernel) */
+   column-split-waves kernel for every n <= 64; 5 the matrix-core pair for
+   FP32 factors with n <= 64, nr <= 8, nx <= 16, n + nr <= 48 / 64 / 80 for
+   n <= 32 / 48 / 64 (FP64 factors and other shapes: the register-tiled
+   kernel) */
 int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant);
 
 /* ---- precision (extension of the reference surface) -------------------------- */
