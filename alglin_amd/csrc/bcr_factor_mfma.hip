@@ -49,6 +49,9 @@ namespace {
 typedef float f4_t __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
+// the most border rows (nr) the matrix-core pair takes
+constexpr int kNrB = 8;
+
 // LDS geometry: NT top slots (pivot positions, NMAX a multiple of 16), NR
 // rest slots (reduced and border rows, n + nr <= RESTMAX)
 template <int NMAX, int RESTMAX, int RS>
@@ -134,20 +137,22 @@ struct RowSrc {
 
 // the raw source values of one row in right-hand tile t (16 columns of the
 // tile's region; clamped addresses, masked when staged)
+// (the source blocks come as values read from the argument record before
+// the kernel's first store: a batched kernel's record sits in memory, and a
+// field read after a store is fetched again, behind it)
 template <typename TS>
-__device__ __forceinline__ void load_rhs_row(TS (&xs)[16], FactorArgs<TS, float> const& a, int t, int tA, int j,
-                                             int jp, RowSrc rs) {
-  const int n = a.n, nr = a.nr, nx = a.nx;
+__device__ __forceinline__ void load_rhs_row(TS (&xs)[16], const TS* sD, const TS* sE, const TS* sB, const TS* sC,
+                                             int n, int nr, int nx, int t, int tA, int j, int jp, RowSrc rs) {
   const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
   const int reg = t < tA ? 0 : t < 2 * tA ? 1 : 2;
   const int c0 = 16 * (reg == 0 ? t : reg == 1 ? t - tA : t - 2 * tA);
   const int width = reg == 2 ? nx : n;
   // region 0: [D_jp | . | C_jp]; region 1: [. | E_j | .]; region 2: [B_jp | B_j | .]
   const bool valid = reg == 0 ? (rs.cls & 5) != 0 : reg == 1 ? rs.cls == 2 : (rs.cls & 3) != 0;
-  const TS* src = reg == 0 ? (rs.cls == 4 ? a.sC + jp * rn : a.sD + jp * nn)
-                : reg == 1 ? a.sE + j * nn
-                           : a.sB + (rs.cls == 1 ? jp : j) * bn;
-  src = valid ? src + rs.off : a.sD + jp * nn;   // a valid address for the rows of zeros
+  const TS* src = reg == 0 ? (rs.cls == 4 ? sC + jp * rn : sD + jp * nn)
+                : reg == 1 ? sE + j * nn
+                           : sB + (rs.cls == 1 ? jp : j) * bn;
+  src = valid ? src + rs.off : sD + jp * nn;   // a valid address for the rows of zeros
   const int stride = reg == 0 && rs.cls == 4 ? nr : n;
 #pragma unroll
   for (int cc = 0; cc < 16; ++cc) xs[cc] = src[(c0 + cc < width ? c0 + cc : width - 1) * stride];
@@ -307,6 +312,12 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
   TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
+  // the record's pointers, read once before the first store (a batched
+  // kernel's record sits in memory: a field read after a store is fetched
+  // again, a dependent memory round trip in front of the load it addresses)
+  const TS *const sD = a.sD, *const sE = a.sE, *const sB = a.sB, *const sC = a.sC;
+  TW *const Dm = a.D, *const Em = a.E, *const Bm = a.B, *const Cm = a.C, *const Fm = a.Fpart;
+  const int32_t* const perm = a.perm;
 
   // this lane's two panel rows as right-hand sources, and the first tile's
   // loads, in flight while the multipliers are staged
@@ -324,7 +335,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   TS xs[RS][16];
   if (w < ntiles) {
 #pragma unroll
-    for (int s = 0; s < RS; ++s) load_rhs_row(xs[s], a, w, tA, j, jp, rsrc[s]);
+    for (int s = 0; s < RS; ++s) load_rhs_row(xs[s], sD, sE, sB, sC, n, nr, nx, w, tA, j, jp, rsrc[s]);
   }
 
   // Ls: top rows (final positions < n) hold -L11 below the diagonal blocks
@@ -346,6 +357,13 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
       const int q0 = 2 * (w + NW * h), qc = q0 < n2 ? q0 : 0;
       tv[h].x = Tj[qc + size_t(kk) * n2];
       tv[h].y = Tj[qc + 1 + size_t(kk) * n2];
+    }
+    // the border rows C_j U^-1 (nr <= kNrB, nr x n column-major), issued with
+    // the T loads; wave 0 stores them
+    TW cv[kNrB];
+    if (nr > 0) {
+#pragma unroll
+      for (int ri = 0; ri < kNrB; ++ri) cv[ri] = Cm[j * rn + (ri < nr ? ri : 0) + size_t(kk) * nr];
     }
 #pragma unroll
     for (int h = 0; h < PAIRS; ++h) {
@@ -370,8 +388,9 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     }
     // border rows (C_j U^-1, nr x n column-major) and the zero padding rows
     if (w == 0 && k < NT) {
-      for (int ri = 0; ri < nr; ++ri)
-        Ls[(NT + n + ri) * LD + k] = keep_if(-a.C[j * rn + ri + size_t(kk) * nr], k < n);
+#pragma unroll
+      for (int ri = 0; ri < kNrB; ++ri)
+        if (ri < nr) Ls[(NT + n + ri) * LD + k] = keep_if(-cv[ri], k < n);
       for (int q = n; q < NT; ++q) Ls[q * LD + k] = TW(0);
       for (int i = n + nr; i < NR; ++i) Ls[(NT + i) * LD + k] = TW(0);
     }
@@ -379,7 +398,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   // the final position of this lane's two rows: LAPACK's swap sequence replayed
   int pos0 = lane, pos1 = lane + 64;
   {
-    const int pv = lane < n ? a.perm[size_t(j) * n + lane] - 1 : 0;
+    const int pv = lane < n ? perm[size_t(j) * n + lane] - 1 : 0;
     for (int c = 0; c < n; ++c) {
       const int P = __builtin_amdgcn_readlane(pv, c);
       pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
@@ -458,7 +477,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     // the next tile's loads fly over this tile's work
     if (t + NW < ntiles) {
 #pragma unroll
-      for (int s = 0; s < RS; ++s) load_rhs_row(xs[s], a, t + NW, tA, j, jp, rsrc[s]);
+      for (int s = 0; s < RS; ++s) load_rhs_row(xs[s], sD, sE, sB, sC, n, nr, nx, t + NW, tA, j, jp, rsrc[s]);
     }
     if (reg == 0 && !a.first) {   // pending C_jp into the border slots
       TW* pe = pend_prv + jp * rn + eb;
@@ -476,7 +495,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     // where this lane's column goes: pivot rows -> block j, reduced rows -> block jp
     const int kc = c0 + cl;
     const bool colok = kc < width;
-    TW* const blk = reg == 0 ? a.D : reg == 1 ? a.E : a.B;
+    TW* const blk = reg == 0 ? Dm : reg == 1 ? Em : Bm;
     const size_t bsz = reg == 2 ? bn : nn;
     const int kcs = colok ? kc : 0;
     TW* const dtop = blk + size_t(j) * bsz + size_t(kcs) * n;
@@ -508,8 +527,8 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     else
       rest_tiles<NB, NR / 16, NT, LD, XLD>(u, Ls, xt, cl, g, n, nr, colok, drest);
     // border rows: C_jp' (region 0), pending C_jpp += (region 1), F partial (region 2)
-    TW* const bdst = reg == 0 ? a.C + jp * rn + eb : reg == 1 ? pend_cur + jpp * rn + eb
-                                                                : a.Fpart + size_t(j) * nr * nx + eb;
+    TW* const bdst = reg == 0 ? Cm + jp * rn + eb : reg == 1 ? pend_cur + jpp * rn + eb
+                                                              : Fm + size_t(j) * nr * nx + eb;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int e = lane + 64 * h;
@@ -548,7 +567,7 @@ constexpr int kRhsWaves = 2;
 // n + nr <= 64 (three); 3: n <= 64, n + nr <= 80 (four, and a third row set
 // for the border rows beyond row 128); all nr <= 8, nx <= 16; 0: not covered
 int mfma_variant(int n, int nr, int nx) {
-  if (n < 1 || nr > 8 || nx > 16) return 0;
+  if (n < 1 || nr > kNrB || nx > 16) return 0;
   if (n <= 32 && n + nr <= 48) return 1;
   if (n <= 48 && n + nr <= 64) return 2;
   if (n <= 64 && n + nr <= 80) return 3;
