@@ -72,6 +72,20 @@ __device__ __forceinline__ f4_t mfma4(f4_t a, f4_t b, f4_t c) {
   return c;
 }
 
+// rows 16 kb + 4 g + i (i < 4) of one column, those below lim: one 16-byte
+// store when the 16-row block lies below lim (a uniform test), single stores
+// in the block lim cuts
+__device__ __forceinline__ void store_block(float* col, int kb, int g, f4_t v, int lim) {
+  const int r0 = 16 * kb + 4 * g;
+  if (16 * kb + 16 <= lim) {
+    __builtin_memcpy(col + r0, &v, 16);   // 4-byte aligned; the hardware takes it
+  } else if (16 * kb < lim) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (r0 + i < lim) col[r0 + i] = v[i];
+  }
+}
+
 // lane `who`'s value of v when K > cc, else 0: a scalar select on the
 // v_readlane result (written out: the compiler would move the value to a
 // VGPR and select there, two vector instructions instead of none)
@@ -119,13 +133,19 @@ __device__ __forceinline__ void rest_tiles(const f4_t (&u)[NB], const float* Ls,
       for (int rt = 0; rt < NRT; ++rt)
         acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], u[kb][i], acc[rt], 0, 0, 0);
   }
+  // rest position q = 16 rt + 4 g + i: reduced row q (< n), else border row q - n
+  if (colok) {
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) store_block(drest, rt, g, acc[rt], n);
+  }
 #pragma unroll
   for (int rt = 0; rt < NRT; ++rt)
+    if (16 * rt + 16 > n && 16 * rt < n + nr) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int q = rt * 16 + 4 * g + i;   // rest position: reduced row q (< n), else border row q - n
-      if (colok && q < n) drest[q] = acc[rt][i];
-      if (q >= n && q < n + nr) xt[cl * XLD + NT + q] = acc[rt][i];
+      for (int i = 0; i < 4; ++i) {
+        const int q = rt * 16 + 4 * g + i;
+        if (q >= n && q < n + nr) xt[cl * XLD + NT + q] = acc[rt][i];
+      }
     }
 }
 
@@ -513,11 +533,10 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
         acc = mfma4(*reinterpret_cast<const f4_t*>(Ls + (kb * 16 + cl) * LD + mb * 16 + 4 * g), u[mb], acc);
       u[kb] = mfma4(*reinterpret_cast<const f4_t*>(Ls + (kb * 16 + cl) * LD + kb * 16 + 4 * g), acc,
                     f4_t{0.f, 0.f, 0.f, 0.f});
+    }
+    if (colok) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int q = kb * 16 + 4 * g + i;
-        if (colok && q < n) dtop[q] = u[kb][i];
-      }
+      for (int kb = 0; kb < NB; ++kb) store_block(dtop, kb, g, u[kb], n);
     }
     // X_rest' = X_rest - [L21; C_j U^-1] U12 (the minus sign is in Ls), over
     // the 16-row tiles the n + nr rest rows occupy (one fewer than NR / 16
