@@ -142,6 +142,18 @@ __device__ __forceinline__ double keep_if(double v, bool keep) {
   return __longlong_as_double(__double_as_longlong(v) & static_cast<long long>(m));
 }
 
+// element i of a device array p through a 32-bit byte offset (i * sizeof(T)
+// < 2^32): with p the same in every lane of the wave the load takes the
+// scalar-base form, one offset register and a 32-bit add per load instead of
+// a 64-bit address, and a global access instead of a flat one (a pointer read
+// from an argument record is otherwise treated as possibly pointing to LDS)
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, unsigned i) {
+  typedef const __attribute__((address_space(1))) char* gbytes;
+  typedef const __attribute__((address_space(1))) T* gptr;
+  return *(gptr)((gbytes)(p) + i * unsigned(sizeof(T)));
+}
+
 // batched refinement: a record whose system has converged (or stalled) is
 // skipped by every kernel of the residual and correction launches
 __device__ __forceinline__ bool skipped(const int32_t* f) { return f != nullptr && *f != 0; }

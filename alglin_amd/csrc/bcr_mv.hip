@@ -45,10 +45,13 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
   if (k0 < N) {
     const bool row = live && k < N && r < n;
     const int kr = row ? k : k0;
-    const TA* Dk = a.D + size_t(kr) * n * n;
-    const TA* Ek = a.E + size_t(kr) * n * n;
-    const TA* Bk = a.B + size_t(kr) * n * nx;
+    // blocks from the wave's first block row on (one base for the wave),
+    // this lane's row through a 32-bit offset
+    const TA* Dk = a.D + size_t(k0) * n * n;
+    const TA* Ek = a.E + size_t(k0) * n * n;
+    const TA* Bk = a.B + size_t(k0) * n * nx;
     const int rr = r < n ? r : 0;
+    const unsigned odk = unsigned(kr - k0) * unsigned(n * n) + rr, obk = unsigned(kr - k0) * unsigned(n * nx) + rr;
     const double xn = (row ? xk : a.x + size_t(k0) * n)[n + rr];   // x_{k+1}
     double acc = 0;
     for (int c0 = 0; c0 < n; c0 += CH) {
@@ -56,8 +59,8 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         const int cc = c0 + i < n ? c0 + i : 0;
-        d[i] = Dk[rr + cc * n];
-        e[i] = Ek[rr + cc * n];
+        d[i] = gload(Dk, odk + unsigned(cc * n));
+        e[i] = gload(Ek, odk + unsigned(cc * n));
       }
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
@@ -72,7 +75,7 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         const int cc = c0 + i < nx ? c0 + i : 0;
-        bb[i] = Bk[rr + cc * n];
+        bb[i] = gload(Bk, obk + unsigned(cc * n));
         w[i] = xb[cc];
       }
 #pragma unroll
@@ -87,15 +90,16 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
   if (nr > 0) {
     const int i = r;
     const bool crow = live && i < nr;
-    const TA* Ck = a.C + size_t(kc) * nr * n;
+    const TA* Ck = a.C + size_t(k0) * nr * n;
     const int ic = crow ? i : 0;
+    const unsigned ock = unsigned(kc - k0) * unsigned(nr * n) + ic;
     double acc = 0;
     for (int c0 = 0; c0 < n; c0 += CH) {
       TA cv[CH];
 #pragma unroll
       for (int t = 0; t < CH; ++t) {
         const int cc = c0 + t < n ? c0 + t : 0;
-        cv[t] = Ck[ic + cc * nr];
+        cv[t] = gload(Ck, ock + unsigned(cc * nr));
       }
 #pragma unroll
       for (int t = 0; t < CH; ++t) {

@@ -47,7 +47,7 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
   constexpr int G = 2 * NMAX, PG = 64 / G;
   const int lane = threadIdx.x & 63;
   const int r = lane % G, base = lane - r;
-  const int p = (int(bid) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
+  const int p = (int(bid) * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6))) * PG + lane / G;
   const bool live = p < a.npairs;
   const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
   const int64_t ldx = a.ldx;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
   TW t[NMAX];   // kept in the factor precision, widened where used
 #pragma unroll
   for (int c = 0; c < NMAX; ++c) {
-    const TW v = Tj[rc + size_t(c < n ? c : 0) * n2];
+    const TW v = gload(Tj, rc + unsigned(c < n ? c : 0) * n2);
     t[c] = keep_if(v, row && c < n);
   }
 
@@ -118,7 +118,7 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int ic = i0 + i < nr ? i0 + i : 0;
-        const TW v = Cj[ic + size_t(crow ? r : 0) * nr];
+        const TW v = gload(Cj, ic + unsigned(crow ? r : 0) * nr);
         ci[i] = TX(keep_if(v, crow && i0 + i < nr));
       }
 #pragma unroll
@@ -145,7 +145,7 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
   constexpr int G = 2 * NMAX, PG = 64 / G;
   const int lane = threadIdx.x & 63;
   const int r = lane % G, base = lane - r;
-  const int p = (int(bid) * 4 + int(threadIdx.x >> 6)) * PG + lane / G;
+  const int p = (int(bid) * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6))) * PG + lane / G;
   const bool live = p < a.npairs;
   const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
   const int64_t ldx = a.ldx;

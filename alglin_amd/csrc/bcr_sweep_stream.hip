@@ -30,10 +30,13 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
+// columns loaded per chunk: one memory latency per chunk
+constexpr int kChunk = 16;
+
 template <typename TW, typename TX>
 __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int bid) {
   const int lane = threadIdx.x & 63;
-  const int p = bid * 4 + int(threadIdx.x >> 6);
+  const int p = bid * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));   // wave-uniform
   if (p >= a.npairs) return;   // uniform over the wavefront
   const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
   int j, jp, jpp;
@@ -56,23 +59,23 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
   auto in_row = [&](int r) { return r < n ? size_t(jp) * n + r : size_t(j) * n + (r - n); };
   auto out_row = [&](int r) { return r < n ? size_t(j) * n + r : size_t(jp) * n + (r - n); };
   const TW* Cj = a.C + size_t(j) * nr * n;
-  // factor columns eight at a time, from clamped addresses: one memory
+  // factor columns kChunk at a time, from clamped addresses: one memory
   // latency per chunk (rows past 2n hold garbage that is never used)
   const int r0c = v0 ? r0 : 0, r1c = v1 ? r1 : 0;
   for (int q = 0; q < nrhs; ++q) {
     TX* x = a.x + size_t(q) * a.ldx;
     TX x0 = v0 ? x[in_row(src0)] : TX(0);
     TX x1 = v1 ? x[in_row(src1)] : TX(0);
-    for (int c0 = 0; c0 < n; c0 += 8) {
-      TW t0[8], t1[8];
+    for (int c0 = 0; c0 < n; c0 += kChunk) {
+      TW t0[kChunk], t1[kChunk];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const size_t cc = size_t(c0 + i < n ? c0 + i : 0) * n2;
-        t0[i] = Tj[r0c + cc];
-        t1[i] = Tj[r1c + cc];
+      for (int i = 0; i < kChunk; ++i) {
+        const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n2;
+        t0[i] = gload(Tj, r0c + cc);
+        t1[i] = gload(Tj, r1c + cc);
       }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < kChunk; ++i) {
         const int c = c0 + i;
         if (c < n) {
           const TX xc = TX(rlane(double(x0), c));
@@ -93,7 +96,7 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
 template <typename TW, typename TX>
 __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int bid) {
   const int lane = threadIdx.x & 63;
-  const int p = bid * 4 + int(threadIdx.x >> 6);
+  const int p = bid * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));   // wave-uniform
   if (p >= a.npairs) return;
   const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
   int j, jp, jpp;
@@ -114,33 +117,33 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
     const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
     const TX vjpp = row ? x[size_t(jpp) * n + lane] : TX(0);
     TX s = row ? x[size_t(j) * n + lane] : TX(0);
-    // D_j, E_j columns eight at a time (one memory latency per chunk)
-    for (int c0 = 0; c0 < n; c0 += 8) {
-      TW d[8], e[8];
+    // D_j, E_j columns kChunk at a time (one memory latency per chunk)
+    for (int c0 = 0; c0 < n; c0 += kChunk) {
+      TW d[kChunk], e[kChunk];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const size_t cc = size_t(c0 + i < n ? c0 + i : 0) * n;
-        d[i] = Dj[rc + cc];
-        e[i] = Ej[rc + cc];
+      for (int i = 0; i < kChunk; ++i) {
+        const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n;
+        d[i] = gload(Dj, rc + cc);
+        e[i] = gload(Ej, rc + cc);
       }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < kChunk; ++i) {
         const int c = c0 + i;
         if (c < n) s -= TX(d[i]) * TX(rlane(double(vjp), c)) + TX(e[i]) * TX(rlane(double(vjpp), c));
       }
     }
     for (int c = 0; c < nx; ++c) s -= TX(row ? Bj[lane + size_t(c) * n] : TW(0)) * x[xbrow + c];
-    // U^{-1}, bottom up: lane c owns the diagonal of column c; columns eight
+    // U^{-1}, bottom up: lane c owns the diagonal of column c; columns kChunk
     // at a time from the top of the remaining range down
-    for (int c1 = n - 1; c1 >= 0; c1 -= 8) {
-      TW u[8];
+    for (int c1 = n - 1; c1 >= 0; c1 -= kChunk) {
+      TW u[kChunk];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < kChunk; ++i) {
         const int c = c1 - i;
-        u[i] = Tj[rc + size_t(c >= 0 ? c : 0) * n2];
+        u[i] = gload(Tj, rc + unsigned(c >= 0 ? c : 0) * n2);
       }
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
+      for (int i = 0; i < kChunk; ++i) {
         const int c = c1 - i;
         if (c >= 0) {
           const TX xc = TX(rlane(double(s * dinv), c));
