@@ -603,10 +603,10 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
     BatchTable<SolveArgs<TW, double>> t{static_cast<const SolveArgs<TW, double>*>(l.args), l.entry, l.first};
     hipStream_t s = class_stream(l.cls, true);
     hipError_t e = hipSuccess;
-    if (l.cls <= 3) {
+    if (l.cls <= 2 || (l.cls == 3 && fwd)) {
       if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, s, &e))
         throw HipError("batch solve: no sweep kernel");
-    } else if (l.cls == 4) {
+    } else if (l.cls <= 4) {   // the 16 < n <= 32 backward too: one wave per pair, columns streamed
       if (!launch_sweep_lu_stream_batched(fwd, t, l.nblocks, l.nmax, s, &e))
         throw HipError("batch solve: no sweep kernel");
     } else {

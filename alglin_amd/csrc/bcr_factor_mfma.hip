@@ -426,10 +426,10 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     }
   }
   __syncthreads();   // Ls loaded
-  if (w == 0 && lane < NT) {
+  if (lane < NT && (lane >> 4) % NW == w) {
     // the inverse of diagonal block lane / 16 of L11 (unit lower), column
-    // lane % 16, by forward substitution in place; rows beyond n are zero,
-    // their inverse rows identity
+    // lane % 16, by forward substitution in place (the blocks dealt out to
+    // the waves); rows beyond n are zero, their inverse rows identity
     const int b = lane >> 4, jj = lane & 15;
     TW y[16];
 #pragma unroll

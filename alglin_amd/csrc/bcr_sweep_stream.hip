@@ -132,7 +132,20 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
         if (c < n) s -= TX(d[i]) * TX(rlane(double(vjp), c)) + TX(e[i]) * TX(rlane(double(vjpp), c));
       }
     }
-    for (int c = 0; c < nx; ++c) s -= TX(row ? Bj[lane + size_t(c) * n] : TW(0)) * x[xbrow + c];
+    // B_j xb, eight columns' loads in flight at a time
+    for (int c0 = 0; c0 < nx; c0 += 8) {
+      TW bv[8];
+      TX xv[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int cc = c0 + i < nx ? c0 + i : 0;
+        bv[i] = gload(Bj, rc + unsigned(cc) * n);
+        xv[i] = x[xbrow + cc];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (c0 + i < nx) s -= TX(keep_if(bv[i], row)) * xv[i];
+    }
     // U^{-1}, bottom up: lane c owns the diagonal of column c; columns kChunk
     // at a time from the top of the remaining range down
     for (int c1 = n - 1; c1 >= 0; c1 -= kChunk) {
