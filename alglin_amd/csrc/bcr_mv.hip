@@ -53,22 +53,30 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
     const int rr = r < n ? r : 0;
     const unsigned odk = unsigned(kr - k0) * unsigned(n * n) + rr, obk = unsigned(kr - k0) * unsigned(n * nx) + rr;
     const double xn = (row ? xk : a.x + size_t(k0) * n)[n + rr];   // x_{k+1}
-    double acc = 0;
+    // the border partial C_k x_k of the same columns rides along (lanes
+    // (sub-row, i) with i < nr; a sub-row k = N has only this part)
+    const bool crow = live && r < nr;
+    const TA* Ck = a.C + size_t(k0) * nr * n;
+    const unsigned ock = unsigned(kc - k0) * unsigned(nr * n) + (crow ? r : 0);
+    double acc = 0, accc = 0;
     for (int c0 = 0; c0 < n; c0 += CH) {
-      TA d[CH], e[CH];
+      TA d[CH], e[CH], cv[CH];
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         const int cc = c0 + i < n ? c0 + i : 0;
         d[i] = gload(Dk, odk + unsigned(cc * n));
         e[i] = gload(Ek, odk + unsigned(cc * n));
+        cv[i] = nr > 0 ? gload(Ck, ock + unsigned(cc * nr)) : TA(0);
       }
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
         const int cc = c0 + i < n ? c0 + i : 0;
         const double u = bcast(xa, cc), v = bcast(xn, cc);
         acc += keep_if(double(d[i]) * u + double(e[i]) * v, c0 + i < n);
+        accc += keep_if(double(cv[i]) * u, c0 + i < n);
       }
     }
+    if (nr > 0 && crow) a.Cpart[size_t(k) * nr + r] = accc;
     for (int c0 = 0; c0 < nx; c0 += CH) {
       TA bb[CH];
       double w[CH];
@@ -86,8 +94,9 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
       a.out[out] = a.b ? a.b[out] - acc : acc;
     }
   }
-  // border partials C_k x_k: lanes (sub-row, i) with i < nr
-  if (nr > 0) {
+  // border partials C_k x_k of a wave that starts at block row N (the
+  // others formed them above): lanes (sub-row, i) with i < nr
+  if (nr > 0 && k0 >= N) {
     const int i = r;
     const bool crow = live && i < nr;
     const TA* Ck = a.C + size_t(k0) * nr * n;
