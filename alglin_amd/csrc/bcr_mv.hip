@@ -58,6 +58,19 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
     const bool crow = live && r < nr;
     const TA* Ck = a.C + size_t(k0) * nr * n;
     const unsigned ock = unsigned(kc - k0) * unsigned(nr * n) + (crow ? r : 0);
+    // the first columns of B_k xb and this row's entry of b, in flight over
+    // the column loop below
+    constexpr int CB = 4;
+    TA bb0[CB];
+    double w0[CB];
+#pragma unroll
+    for (int i = 0; i < CB; ++i) {
+      const int cc = i < nx ? i : 0;
+      bb0[i] = nx > 0 ? gload(Bk, obk + unsigned(cc * n)) : TA(0);
+      w0[i] = nx > 0 ? xb[cc] : 0.0;
+    }
+    const size_t out = size_t(kr) * n + rr;
+    const double bval = a.b ? a.b[out] : 0.0;
     double acc = 0, accc = 0;
     for (int c0 = 0; c0 < n; c0 += CH) {
       TA d[CH], e[CH], cv[CH];
@@ -77,7 +90,9 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
       }
     }
     if (nr > 0 && crow) a.Cpart[size_t(k) * nr + r] = accc;
-    for (int c0 = 0; c0 < nx; c0 += CH) {
+#pragma unroll
+    for (int i = 0; i < CB; ++i) acc += keep_if(double(bb0[i]) * w0[i], i < nx);
+    for (int c0 = CB; c0 < nx; c0 += CH) {
       TA bb[CH];
       double w[CH];
 #pragma unroll
@@ -89,10 +104,7 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
 #pragma unroll
       for (int i = 0; i < CH; ++i) acc += keep_if(double(bb[i]) * w[i], c0 + i < nx);
     }
-    if (row) {
-      const size_t out = size_t(k) * n + r;
-      a.out[out] = a.b ? a.b[out] - acc : acc;
-    }
+    if (row) a.out[out] = a.b ? bval - acc : acc;
   }
   // border partials C_k x_k of a wave that starts at block row N (the
   // others formed them above): lanes (sub-row, i) with i < nr
