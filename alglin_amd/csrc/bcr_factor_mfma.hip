@@ -479,8 +479,17 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     for (int s = 0; s < RS; ++s) {
       const int cls = rsrc[s].cls;
       const bool valid = reg == 0 ? (cls & 5) != 0 : reg == 1 ? cls == 2 : (cls & 3) != 0;
+      // a whole tile (a uniform test): one row mask for all columns (not for
+      // NB = 2, where the second copy of the loop would cost a wave per SIMD)
+      if (NB > 2 && c0 + 16 <= width) {
+        int m = valid ? -1 : 0;
+        asm volatile("" : "+v"(m));
 #pragma unroll
-      for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = keep_if(TW(xs[s][cc]), valid && c0 + cc < width);
+        for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = __int_as_float(__float_as_int(TW(xs[s][cc])) & m);
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = keep_if(TW(xs[s][cc]), valid && c0 + cc < width);
+      }
     }
     // the border rows' extra terms, nr x 16 contiguous values each (element
     // e = border row e % nr, tile column e / nr): pending C_jp (region 0, read
