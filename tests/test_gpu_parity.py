@@ -177,6 +177,31 @@ def test_large_n32_matches_oracle(oracle_mod):
     assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-13
 
 
+@pytest.mark.parametrize("n,nr,nx", [(40, 3, 2), (48, 8, 16), (33, 2, 9), (20, 5, 9), (6, 1, 1), (8, 0, 0)])
+def test_device_residual_shapes(n, nr, nx):
+    """The GPU residual r = b - A x (BABD_BorderedCR.cc:2111-2166): block rows
+    with the border partials C_k x_k formed in the same column loop, the
+    first B columns and b loaded ahead, several block rows per wave for small
+    n; against the host product of the same blocks."""
+    from alglin_amd.bcr import DeviceArray, packed_blocks
+    N = 37
+    qr = max(min(4, n), nx - nr)
+    qx = qr + nr - nx   # a square last block (the LU last-block solver)
+    s = systems.gaussian_system(N, n, qr, qx, nr, nx, shift=4.0, seed=n + nr)
+    dev = {k: DeviceArray.from_host(v) for k, v in packed_blocks(s).items()}
+    g = BorderedCR()
+    g.allocate(N, n, qr, qx, nr, nx)
+    g.bind_device(dev)
+    assert g.factorize_device(sync=True), g.last_error()
+    nrows, ncols = systems.dims(s)
+    rng = np.random.default_rng(n)
+    x, b = rng.standard_normal(ncols), rng.standard_normal(nrows)
+    X, bd, r = DeviceArray.from_host(x), DeviceArray.from_host(b), DeviceArray(nrows)
+    g.residual_device(X, bd, r)
+    ref = b - systems.mv(s, x)
+    assert np.abs(r.to_host() - ref).max() <= 1e-12 * (1.0 + np.abs(ref).max())
+
+
 @pytest.mark.parametrize("n,nx", [(32, 4), (24, 8), (12, 2), (6, 1)])
 def test_full_size_residual(n, nx):
     """Size-independent property at the large single-system size: the FP64
