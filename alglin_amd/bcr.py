@@ -51,6 +51,13 @@ def lib():
             "alglin_bcr_check_matrix": (ci, [vp]),
             "alglin_bcr_last_error": (cs, [vp]),
             "alglin_bcr_info": (cs, [vp]),
+            "alglin_bcr_set_num_parallel_block": (ci, [vp, i32]),
+            "alglin_bcr_set_debug": (ci, [vp, ci]),
+            "alglin_bcr_used_parallel_block": (i32, [vp]),
+            "alglin_bcr_dup": (ci, [vp, vp]),
+            "alglin_bcr_t_solve": (ci, [vp, vp]),
+            "alglin_bcr_matlab_script": (i64, [vp, ctypes.c_char_p, i64]),
+            "alglin_bcr_get_packed": (ci, [vp] * 8),
             "alglin_bcr_bind_device": (ci, [vp] * 8),
             "alglin_bcr_factorize_device": (ci, [vp, ci]),
             "alglin_bcr_solve_device": (ci, [vp, vp, i32, i64, ci]),
@@ -243,6 +250,44 @@ class BorderedCR:
     def check_matrix(self):
         self._check(lib().alglin_bcr_check_matrix(self._h), "check_matrix")
 
+    def set_debug(self, on: bool):
+        self._check(lib().alglin_bcr_set_debug(self._h, int(on)), "set_debug")
+
+    def set_num_parallel_block(self, npb: int):
+        """Elimination order of the reference with npb threads (cc:139-158)."""
+        self._check(lib().alglin_bcr_set_num_parallel_block(self._h, npb), "set_num_parallel_block")
+
+    def used_parallel_block(self) -> int:
+        return int(lib().alglin_bcr_used_parallel_block(self._h))
+
+    def dup(self, other: "BorderedCR"):
+        self._check(lib().alglin_bcr_dup(self._h, other._h), "dup")
+
+    def t_solve(self, b):
+        x = np.array(b, self.dtype)
+        self._check(lib().alglin_bcr_t_solve(self._h, _p(x)), "t_solve")
+        return x
+
+    def matlab_script(self) -> str:
+        n = lib().alglin_bcr_matlab_script(self._h, None, 0)
+        if n < 0:
+            raise AlglinError(f"matlab_script: {self.last_error()}")
+        buf = ctypes.create_string_buffer(int(n))
+        lib().alglin_bcr_matlab_script(self._h, buf, n)
+        return buf.value.decode()
+
+    def packed(self) -> dict:
+        """Host copies of the loaded blocks (packed layout)."""
+        d = self.dims()
+        N, n, qr, qx, nr, nx, Nc = d["N"], d["n"], d["qr"], d["qx"], d["nr"], d["nx"], d["Nc"]
+        out = dict(D=np.zeros(N * n * n, self.dtype), E=np.zeros(N * n * n, self.dtype),
+                   B=np.zeros(max(1, N * n * nx), self.dtype), C=np.zeros(max(1, (N + 1) * nr * n), self.dtype),
+                   H=np.zeros((n + qr) * Nc, self.dtype), Cq=np.zeros(max(1, nr * qx), self.dtype),
+                   F=np.zeros(max(1, nr * nx), self.dtype))
+        self._check(lib().alglin_bcr_get_packed(self._h, *[_p(out[k]) for k in ("D", "E", "B", "C", "H", "Cq", "F")]),
+                    "get_packed")
+        return out
+
     def factors(self):
         d = self.dims()
         N, n, nr, nx = d["N"], d["n"], d["nr"], d["nx"]
@@ -282,10 +327,13 @@ class BorderedCR:
         self._check(lib().alglin_bcr_set_fallback(self._h, int(on)), "set_fallback")
 
     def refine_report(self) -> dict:
-        out = np.zeros(4)
+        """status: 0 converged, 1 stalled, 2 max_iter, 3 fallback factorization
+        failed, 4 FP64 QRP fallback reached tol, 5 FP64 residual floor (the
+        componentwise backward error is at roundoff), 6 fallback missed tol."""
+        out = np.zeros(5)
         self._check(lib().alglin_bcr_refine_report(self._h, _p(out)), "refine_report")
         return dict(iterations=int(out[0]), rel_residual=float(out[1]), first_residual=float(out[2]),
-                    status=int(out[3]))
+                    status=int(out[3]), backward_error=float(out[4]))
 
     def set_timing(self, on=True):
         self._check(lib().alglin_bcr_set_timing(self._h, int(on)), "set_timing")

@@ -108,6 +108,17 @@ void BatchEngine<TA, TW>::join() {
   hip_check(hipStreamWaitEvent(st_, join_, 0), "join");
 }
 
+// the launch of a system's class takes a panel in LDS only when no register
+// kernel covers the shape: LU with n <= 32 (wave kernels), LU up to n = 64
+// through the column-split or register-tiled kernels; QR and QRP always
+template <typename TA, typename TW>
+bool BatchEngine<TA, TW>::panel_fits(SysDims const& d, size_t lds_limit) {
+  const int c = factor_class(d);
+  if (c < 4) return true;
+  if (c == 4 && (pair_lu_cols_fits(d.n, d.nr, d.nx) || pair_lu_tiled_fits(d.n, d.nr, d.nx))) return true;
+  return size_t(2 * d.n + d.nr) * (3 * d.n + d.nx) * sizeof(TW) <= lds_limit;
+}
+
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
   if (dims.empty()) throw HipError("batch plan: no systems");
@@ -118,9 +129,7 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
     if (d.n > 64 || d.nr > 16) throw HipError(id + "block size n <= 64 and nr <= 16 are supported");
     if (d.Nr() != d.Nc()) throw HipError(id + "the last-block LU needs qr + nr == qx + nx");
     if (d.method != kLU && d.method != kQR && d.method != kQRP) throw HipError(id + "method must be LU, QR or QRP");
-    const int c = factor_class(d);
-    if (c >= 4 && size_t(2 * d.n + d.nr) * (3 * d.n + d.nx) * sizeof(TW) > lds_limit_)
-      throw HipError(id + "pair panel does not fit in LDS");
+    if (!panel_fits(d, lds_limit_)) throw HipError(id + "pair panel does not fit in LDS");
   }
   dims_ = dims;
   const int ns = nsys();
@@ -251,8 +260,7 @@ void BatchEngine<TA, TW>::set_method(int s, int method) {
   if (method != kLU && method != kQR && method != kQRP) throw HipError("set_method: method must be LU, QR or QRP");
   SysDims d = dims_[s];
   d.method = method;
-  if (factor_class(d) >= 4 && size_t(2 * d.n + d.nr) * (3 * d.n + d.nx) * sizeof(TW) > lds_limit_)
-    throw HipError("set_method: pair panel does not fit in LDS");
+  if (!panel_fits(d, lds_limit_)) throw HipError("set_method: pair panel does not fit in LDS");
   if (dims_[s].method != method) {
     dims_[s].method = method;
     invalidate();
@@ -803,6 +811,7 @@ void BatchSolver::drop_graphs() {
     if (*g) { (void)hipGraphExecDestroy(*g); *g = nullptr; }
   g_.B = nullptr;
   g_.X = nullptr;
+  g_eng_ = nullptr;
 }
 
 hipStream_t BatchSolver::stream() const { return mixed_ ? e32_->stream() : e64_->stream(); }
@@ -820,7 +829,23 @@ void BatchSolver::plan(std::vector<SysDims> const& dims) {
   drop_graphs();
   if (mixed_) {
     e32_->plan(dims);
-    if (e64_) e64_->plan(dims);
+    if (e64_) {
+      e64_->plan(dims);
+    } else {
+      // the FP64 fallback (LU, then QRP) is planned when first needed; its
+      // shapes are checked now so that a batch it cannot take fails here
+      // rather than after the FP32 work
+      const size_t lds = e32_->lds_limit();
+      for (size_t k = 0; k < dims.size(); ++k) {
+        SysDims d = dims[k];
+        for (int m : {int(kLU), int(kQRP)}) {
+          d.method = m;
+          if (!BatchEngine<double, double>::panel_fits(d, lds))
+            throw HipError("batch plan: system " + std::to_string(k) +
+                           ": the FP64 fallback's pair panel does not fit in LDS");
+        }
+      }
+    }
   } else {
     e64_->plan(dims);
   }
@@ -897,7 +922,8 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
                          BatchReport& rep, int giveup) {
   const int ns = eng.nsys();
   hipStream_t st = eng.stream();
-  const bool graph = g_.res && g_.corr;
+  // the captured lists belong to one engine: the FP64 fallback engine runs eager
+  const bool graph = g_.res && g_.corr && g_eng_ == static_cast<const void*>(&eng);
   std::vector<int32_t> hd(ns);
   for (int s = 0; s < ns; ++s) hd[s] = done[s] ? 1 : 0;
   const size_t i4 = sizeof(int32_t) * size_t(ns), d8 = sizeof(double) * size_t(ns);
@@ -947,6 +973,33 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
   if (narrowed) eng.widen();
 }
 
+// the factor + first solve, the residual + norms and the correction launch
+// lists of the current tables, captured once per (B, X) pair
+template <typename E>
+void BatchSolver::ensure_graphs(E& eng, double const* B, double* X) {
+  if (g_.B == B && g_.X == X && g_.fs && g_eng_ == static_cast<const void*>(&eng)) return;
+  drop_graphs();
+  hipStream_t st = eng.stream();
+  const int64_t nx = eng.total(kbX);
+  eng.prepare(X, B, R_.p);   // no allocation inside a capture
+  g_.fs = capture(st, [&] {
+    hip_check(hipMemcpyAsync(X, B, size_t(nx) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
+    eng.factorize_async();
+    eng.solve_async(X);
+  });
+  g_.res = capture(st, [&] {
+    eng.residual_async(X, B, R_.p);
+    eng.norms_async(B, R_.p, norms_.p);
+  });
+  g_.corr = capture(st, [&] {
+    eng.solve_async(R_.p);
+    eng.add_active_async(R_.p, X);
+  });
+  g_.B = B;
+  g_.X = X;
+  g_eng_ = &eng;
+}
+
 template <typename E>
 BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -957,33 +1010,14 @@ BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int
     drop_graphs();
     eng.set_active(std::vector<char>(ns, 1));
   }
-  auto factor_solve = [&] {
-    hip_check(hipMemcpyAsync(X, B, size_t(nx) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
-    eng.factorize_async();
-    eng.solve_async(X);
-  };
-  auto residual = [&] {
-    eng.residual_async(X, B, R_.p);
-    eng.norms_async(B, R_.p, norms_.p);
-  };
-  auto correct = [&] {
-    eng.solve_async(R_.p);
-    eng.add_active_async(R_.p, X);
-  };
   if (use_graph) {
-    if (g_.B != B || g_.X != X || !g_.fs) {
-      drop_graphs();
-      eng.prepare(X, B, R_.p);   // no allocation inside a capture
-      g_.fs = capture(st, factor_solve);
-      g_.res = capture(st, residual);
-      g_.corr = capture(st, correct);
-      g_.B = B;
-      g_.X = X;
-    }
+    ensure_graphs(eng, B, X);
     hip_check(hipGraphLaunch(g_.fs, st), "graph factor+solve");
   } else {
     drop_graphs();
-    factor_solve();
+    hip_check(hipMemcpyAsync(X, B, size_t(nx) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
+    eng.factorize_async();
+    eng.solve_async(X);
   }
   status_.assign(ns, 0);
   eng.status(status_.data());
@@ -1009,7 +1043,6 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
                                         float threshold, bool fallback) {
   if (!mixed_) throw HipError("solve_selected: needs the mixed batch solver");
   const auto t0 = std::chrono::steady_clock::now();
-  drop_graphs();
   auto& eng = *e32_;
   const int ns = eng.nsys();
   hipStream_t st = eng.stream();
@@ -1020,14 +1053,30 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
   chose_qrp_.assign(ns, 0);
   fell_back_.assign(ns, 0);
   feat_.assign(size_t(ns) * 12, 0.f);
-  // 1. FP32 LU everywhere, first solve and its residual
-  for (int s = 0; s < ns; ++s) eng.set_method(s, kLU);
-  eng.set_active(std::vector<char>(ns, 1));
-  hip_check(hipMemcpyAsync(X, B, size_t(eng.total(kbX)) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
-  eng.factorize_async();
-  eng.solve_async(X);
-  eng.residual_async(X, B, R_.p);
-  eng.norms_async(B, R_.p, norms_.p);
+  // 1. FP32 LU everywhere, first solve and its residual.  With the FP64
+  // fallback the launch lists stay those of every system with LU, so they are
+  // captured once and replayed (like solve()); a previous call that moved
+  // systems to QRP or narrowed the set rebuilds them
+  bool all_lu = int(eng.active().size()) == ns;
+  for (int s = 0; s < ns && all_lu; ++s) all_lu = eng.method(s) == kLU;
+  if (!all_lu) {
+    drop_graphs();
+    for (int s = 0; s < ns; ++s) eng.set_method(s, kLU);
+    eng.set_active(std::vector<char>(ns, 1));
+  }
+  const bool use_graph = fallback && graphs_;
+  if (use_graph) {
+    ensure_graphs(eng, B, X);
+    hip_check(hipGraphLaunch(g_.fs, st), "graph factor+solve");
+    hip_check(hipGraphLaunch(g_.res, st), "graph residual");
+  } else {
+    drop_graphs();
+    hip_check(hipMemcpyAsync(X, B, size_t(eng.total(kbX)) * sizeof(double), hipMemcpyDeviceToDevice, st), "copy b");
+    eng.factorize_async();
+    eng.solve_async(X);
+    eng.residual_async(X, B, R_.p);
+    eng.norms_async(B, R_.p, norms_.p);
+  }
   eng.status(status_.data());
   // 2. features (always: they are also the selector's training inputs), selector
   std::vector<char> direct(ns, 0);   // flagged systems sent straight to the FP64 path
@@ -1054,6 +1103,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       // they would stall in are the costly tail of the refinement
       direct = mask;
     } else if (nq > 0) {
+      drop_graphs();   // the launch lists change with the methods
       for (int s = 0; s < ns; ++s)
         if (mask[s]) eng.set_method(s, kQRP);
       eng.set_active(mask);

@@ -58,6 +58,11 @@ class BatchEngine {
   std::vector<int> const& active() const { return active_; }
   // pair elimination method of system s (kLU, kQR, kQRP)
   void set_method(int s, int method);
+  int method(int s) const { return dims_[size_t(s)].method; }
+  size_t lds_limit() const { return lds_limit_; }
+  // false when the pair panel of d would need LDS beyond lds_limit (the
+  // shapes no register kernel of its class covers)
+  static bool panel_fits(SysDims const& d, size_t lds_limit);
   SysDims const& dims(int s) const { return dims_[s]; }
   int64_t total(int kind) const { return tot_[kind]; }
   int64_t offset(int kind, int s) const { return off_[kind][s]; }
@@ -266,6 +271,9 @@ class BatchSolver {
 
  private:
   template <typename E> BatchReport run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph);
+  template <typename E> void ensure_graphs(E& eng, double const* B, double* X);
+  bool graphs_ = true;   // solve_selected replays captured launch lists
+  const void* g_eng_ = nullptr;   // the engine the captured lists launch on
   bool mixed_;
   BatchEngine<double, double>* e64_ = nullptr;
   BatchEngine<double, float>* e32_ = nullptr;

@@ -42,6 +42,9 @@ struct FactorArgs {
   TW* Cpend;       // [2][N+1][nr*n]
   TW* Fpart;       // [N][nr*nx]
   int32_t* status; // set to 1 when a pivot/diagonal is exactly zero
+  // partitioned elimination order: (jp, j, jpp) of pair p at pairs[3p..3p+2];
+  // nullptr = pure cyclic reduction over [0, N) at stride L
+  const int32_t* pairs = nullptr;
 };
 
 // Per-level arguments of the forward / backward sweeps.
@@ -56,6 +59,7 @@ struct SolveArgs {
   const TW* Tm; const int32_t* perm;
   TX* Xpart;          // [N][nr*nrhs]
   const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
+  const int32_t* pairs = nullptr;  // as FactorArgs::pairs
 };
 
 // Arguments of the last-block assembly.
@@ -173,6 +177,20 @@ __device__ __forceinline__ void pair_index(int p, int L, int N, int& j, int& jp,
   j = L + 2 * L * p;
   jp = j - L;
   jpp = min(j + L, N);
+}
+
+// pair p of a launch: from the launch's pair table when it has one (the
+// partitioned order of BABD_BorderedCR.cc:1072-1080 and 1181-1190: block
+// ranges, then the chain of their end blocks), else pure cyclic reduction
+template <typename Args>
+__device__ __forceinline__ void pair_of(Args const& a, int p, int& j, int& jp, int& jpp) {
+  if (a.pairs != nullptr) {
+    jp = a.pairs[3 * p];
+    j = a.pairs[3 * p + 1];
+    jpp = a.pairs[3 * p + 2];
+  } else {
+    pair_index(p, a.L, a.N, j, jp, jpp);
+  }
 }
 
 }  // namespace bcr

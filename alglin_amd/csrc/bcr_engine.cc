@@ -26,6 +26,7 @@ void hip_check(hipError_t e, char const* what) {
 template <typename T>
 void DevBuf<T>::resize(size_t c) {
   if (c <= count && p != nullptr) return;
+  if (borrowed) throw HipError("working array of " + std::to_string(c) + " elements exceeds its fixed slice");
   release();
   if (c == 0) return;
   hip_check(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)), "hipMalloc");
@@ -33,10 +34,19 @@ void DevBuf<T>::resize(size_t c) {
 }
 
 template <typename T>
+void DevBuf<T>::borrow(T* ptr, size_t c) {
+  release();
+  p = ptr;
+  count = c;
+  borrowed = true;
+}
+
+template <typename T>
 void DevBuf<T>::release() {
-  if (p) (void)hipFree(p);
+  if (p && !borrowed) (void)hipFree(p);
   p = nullptr;
   count = 0;
+  borrowed = false;
 }
 
 template struct DevBuf<double>;
@@ -47,6 +57,16 @@ template struct DevBuf<char>;
 template <typename TA, typename TW>
 Engine<TA, TW>::Engine() {
   hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
+  int dev = 0, lds = 0, cus = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    lds_limit_ = size_t(lds);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+    max_chunks_ = 4 * cus;
+}
+
+template <typename TA, typename TW>
+Engine<TA, TW>::Engine(hipStream_t external) : st_(external), own_stream_(false) {
   int dev = 0, lds = 0, cus = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
@@ -134,8 +154,167 @@ void Engine<TA, TW>::allocate(Dims const& d) {
   Cpart_.resize((N + 1) * std::max<size_t>(1, nr));
   Csum_.resize(std::max<size_t>(1, nr));
   hip_check(hipStreamSynchronize(st_), "allocate sync");
-  Lmax_ = 1;
-  while (Lmax_ < d.N) Lmax_ *= 2;
+  build_schedule();
+}
+
+namespace {
+// the slices of allocate_in, in order: element counts and element sizes
+struct Carve {
+  size_t off = 0;
+  template <typename T>
+  T* take(char* base, size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += std::max<size_t>(1, count) * sizeof(T);
+    return p;
+  }
+};
+
+template <typename TW, typename F>
+void carve_all(Dims const& d, int nrhs, int max_chunks, F&& put) {
+  const size_t N = size_t(d.N), n = size_t(d.n), nr = size_t(d.nr), nx = size_t(d.nx);
+  const size_t m = nr * size_t(std::max(nrhs, 1));
+  put(0, N * n * n);                 // wD
+  put(1, N * n * n);                 // wE
+  put(2, N * n * nx);                // wB
+  put(3, (N + 1) * nr * n);          // wC
+  put(4, N * tsize(d.n));            // Tm
+  put(5, 2 * (N + 1) * nr * n);      // Cpend
+  put(6, N * nr * nx);               // Fpart
+  put(7, size_t(max_chunks) * std::max<size_t>(1, nr * nx));   // tmpw
+  put(8, N * n);                     // perm (int32)
+  put(9, size_t(std::max(d.Nr(), d.Nc())));   // ipiv (int32)
+  put(10, 4);                        // status (int32)
+  put(11, std::max<size_t>(1, nr * nx));   // Feff
+  put(12, size_t(d.Nr()) * d.Nc());  // Hm
+  put(13, size_t(max_chunks) * std::max<size_t>(1, std::max(nr * kRhsChunk, m)));   // tmpd
+  put(14, (N + 1) * std::max<size_t>(1, nr));   // Cpart
+  put(15, std::max<size_t>(1, nr));  // Csum
+  put(16, std::max<size_t>(1, N * m));   // Xpartd
+  (void)sizeof(TW);
+}
+}  // namespace
+
+template <typename TA, typename TW>
+size_t Engine<TA, TW>::bytes_in(Dims const& d, int nrhs, int max_chunks) {
+  Carve c;
+  carve_all<TW>(d, nrhs, max_chunks, [&](int k, size_t cnt) {
+    if (k <= 7) c.take<TW>(nullptr, cnt);
+    else if (k <= 10) c.take<int32_t>(nullptr, cnt);
+    else c.take<double>(nullptr, cnt);
+  });
+  return (c.off + 255) & ~size_t(255);
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::allocate_in(Dims const& d, int nrhs, char* base, size_t bytes) {
+  const size_t need = bytes_in(d, nrhs, max_chunks_);
+  if (need > bytes)
+    throw HipError("working set of " + std::to_string(need) + " bytes exceeds the " + std::to_string(bytes) +
+                   "-byte scratch array");
+  d_ = d;
+  Carve c;
+  DevBuf<TW>* tw[] = {&wD_, &wE_, &wB_, &wC_, &Tm_, &Cpend_, &Fpart_, &tmpw_};
+  DevBuf<int32_t>* ti[] = {&perm_, &ipiv_, &status_};
+  DevBuf<double>* td[] = {&Feff_, &Hm_, &tmpd_, &Cpart_, &Csum_, &Xpartd_};
+  carve_all<TW>(d, nrhs, max_chunks_, [&](int k, size_t cnt) {
+    if (k <= 7) tw[k]->borrow(c.take<TW>(base, cnt), std::max<size_t>(1, cnt));
+    else if (k <= 10) ti[k - 8]->borrow(c.take<int32_t>(base, cnt), std::max<size_t>(1, cnt));
+    else td[k - 11]->borrow(c.take<double>(base, cnt), std::max<size_t>(1, cnt));
+  });
+  // every other array stays empty and refuses to grow: a path that would
+  // need one (QR panels in global scratch, the SVD last block, a pair table)
+  // fails instead of allocating
+  for (DevBuf<TW>* b : {&gscratch_, &Xpartw_}) b->borrow(nullptr, 0);
+  for (DevBuf<int32_t>* b : {&jpiv_, &svd_rank_, &pairtab_}) b->borrow(nullptr, 0);
+  for (DevBuf<double>* b : {&tau_, &Hm2_, &svdW_, &svdVj_, &svdU_, &svdV_, &svdS_}) b->borrow(nullptr, 0);
+  // pure cyclic reduction: the schedule needs no pair table on the device
+  want_P_ = 1;
+  build_schedule();
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::set_partition(int P) {
+  want_P_ = std::max(1, P);
+  if (d_.N > 0) build_schedule();
+}
+
+// The launch list of one factorization (and, in the same order, of the
+// forward sweep; reversed for the backward sweep).
+template <typename TA, typename TW>
+void Engine<TA, TW>::build_schedule() {
+  const int N = d_.N;
+  sched_.clear();
+  copy_rows_.clear();
+  copy_cols_.clear();
+  // partition of cc:139-158: shrink P until 2P <= N
+  int P = want_P_, red = 2 * want_P_;
+  while (red > N) { red -= 2; --P; }
+  if (P <= 1) {
+    used_P_ = 1;
+    iblock_ = {0, N};
+    for (int L = 1; L < N; L *= 2) {
+      Level l;
+      int lg = 0;
+      for (int t = L; t > 1; t >>= 1) ++lg;
+      l.L = L; l.npairs = (N + L - 1) / (2 * L); l.slot = lg & 1; l.first = L == 1;
+      sched_.push_back(l);
+    }
+    // blocks the first level does not touch: row N-1 when N is odd, C_N
+    if (N > 1 && N % 2 == 1) { copy_rows_.push_back(N - 1); copy_cols_.push_back(N - 1); }
+    copy_cols_.push_back(N);
+    return;
+  }
+  used_P_ = P;
+  iblock_.assign(size_t(2 * P), 0);
+  iblock_[0] = 0;
+  iblock_[1] = N / P;
+  for (int t = 1; t < P; ++t) {
+    iblock_[2 * t] = iblock_[2 * t - 1] + 1;
+    iblock_[2 * t + 1] = int((int64_t(t + 1) * N) / P);
+  }
+  const int M = 2 * P - 1;   // reduced chain: rows iblock_[0..M-1], last column iblock_[M] = N
+  std::vector<int32_t> tab;
+  auto slot_of = [](int k) { int lg = 0; for (int t = k; t > 1; t >>= 1) ++lg; return lg & 1; };
+  // ranges: every range's level k in one launch (cc:1072-1080)
+  int maxlen = 0;
+  for (int t = 0; t < P; ++t) maxlen = std::max(maxlen, iblock_[2 * t + 1] - iblock_[2 * t]);
+  for (int k = 1; k < maxlen; k *= 2) {
+    Level l;
+    l.L = k; l.slot = slot_of(k); l.first = k == 1; l.off = int64_t(tab.size());
+    for (int t = 0; t < P; ++t) {
+      const int ib = iblock_[2 * t], eb = iblock_[2 * t + 1];
+      for (int j = ib + k; j < eb; j += 2 * k) {
+        tab.push_back(j - k); tab.push_back(j); tab.push_back(std::min(j + k, eb));
+        ++l.npairs;
+      }
+    }
+    if (l.npairs > 0) sched_.push_back(l);
+  }
+  // the reduced chain (cc:1181-1190): jp = iblock[jj-k], j = iblock[jj],
+  // jpp = iblock[min(jj+k, M)]; the range-end blocks' pending border terms
+  // are read as j at its first level
+  for (int k = 1; k < M; k *= 2) {
+    Level l;
+    l.L = k; l.slot = slot_of(k); l.first = false; l.off = int64_t(tab.size());
+    for (int jj = k; jj < M; jj += 2 * k) {
+      tab.push_back(iblock_[jj - k]); tab.push_back(iblock_[jj]); tab.push_back(iblock_[std::min(jj + k, M)]);
+      ++l.npairs;
+    }
+    sched_.push_back(l);
+  }
+  // rows and C blocks the first launch does not read from A are copied
+  std::vector<char> row(size_t(N), 0), col(size_t(N) + 1, 0);
+  if (!sched_.empty() && sched_[0].first)
+    for (int p = 0; p < sched_[0].npairs; ++p) {
+      const int jp = tab[size_t(sched_[0].off) + 3 * p], j = tab[size_t(sched_[0].off) + 3 * p + 1];
+      row[jp] = row[j] = 1;
+      col[jp] = col[j] = 1;
+    }
+  for (int r = 0; r < N; ++r) if (!row[r]) copy_rows_.push_back(r);
+  for (int c = 0; c <= N; ++c) if (!col[c]) copy_cols_.push_back(c);
+  pairtab_.resize(std::max<size_t>(1, tab.size()));
+  hip_check(hipMemcpy(pairtab_.p, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice), "pair table");
 }
 
 template <typename TA, typename TW>
@@ -163,16 +342,16 @@ void Engine<TA, TW>::bind_device(TA const* D, TA const* E, TA const* B, TA const
 }
 
 template <typename TA, typename TW>
-void Engine<TA, TW>::level_factor(int L, bool first) {
+void Engine<TA, TW>::level_factor(Level const& l) {
   const int N = d_.N;
-  const int npairs = (N + L - 1) / (2 * L);
-  int lg = 0;
-  for (int t = L; t > 1; t >>= 1) ++lg;
+  const int npairs = l.npairs;
+  const bool first = l.first;
   auto fill = [&](auto& a) {
     a.N = N; a.n = d_.n; a.nr = d_.nr; a.nx = d_.nx;
-    a.L = L; a.npairs = npairs; a.slot_cur = lg & 1; a.first = first ? 1 : 0;
+    a.L = l.L; a.npairs = npairs; a.slot_cur = l.slot; a.first = first ? 1 : 0;
     a.D = wD_.p; a.E = wE_.p; a.B = wB_.p; a.C = wC_.p;
     a.Tm = Tm_.p; a.perm = perm_.p; a.Cpend = Cpend_.p; a.Fpart = Fpart_.p; a.status = status_.p;
+    a.pairs = l.off >= 0 ? pairtab_.p + l.off : nullptr;
   };
   auto run = [&](auto& a) {
     using A = std::remove_reference_t<decltype(a)>;
@@ -230,21 +409,21 @@ void Engine<TA, TW>::factorize_async() {
   hip_check(launch_convert<int32_t, int32_t>(nullptr, status_.p, 4, st_), "status");
   // pending border contributions accumulate per factorization
   timed(kReduce, [&] { hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending"); });
-  // blocks the first level does not touch: row N-1 when N is odd, C_N
-  if (N % 2 == 1) {
-    const size_t k = size_t(N - 1);
+  // rows and border blocks the first launch does not read from A (row N-1
+  // when N is odd and C_N in pure cyclic reduction; the range ends too when
+  // partitioned) go to the working arrays as they are
+  for (int r : copy_rows_) {
+    const size_t k = size_t(r);
     hip_check(launch_convert<TA, TW>(pD_ + k * nn, wD_.p + k * nn, int64_t(nn), st_), "copy D");
     hip_check(launch_convert<TA, TW>(pE_ + k * nn, wE_.p + k * nn, int64_t(nn), st_), "copy E");
     hip_check(launch_convert<TA, TW>(pB_ + k * bn, wB_.p + k * bn, int64_t(bn), st_), "copy B");
+  }
+  for (int c : copy_cols_) {
+    const size_t k = size_t(c);
     hip_check(launch_convert<TA, TW>(pC_ + k * rn, wC_.p + k * rn, int64_t(rn), st_), "copy C");
   }
-  hip_check(launch_convert<TA, TW>(pC_ + size_t(N) * rn, wC_.p + size_t(N) * rn, int64_t(rn), st_), "copy CN");
-  int L = 1;
-  bool first = true;
-  while (L < N) {
-    timed(kPair, [&] { level_factor(L, first); });
-    first = false;
-    L *= 2;
+  for (Level const& l : sched_) {
+    timed(kPair, [&] { level_factor(l); });
     ++stats_.levels;
   }
   // F_eff = F + sum_j Fpart[j]  (cc:823-829 / 1146-1154)
@@ -339,10 +518,10 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
     tmp = reinterpret_cast<TX*>(tmpw_.p);
   }
   a.Xpart = Xpart;
-  int L = 1;
-  while (L < N) {
-    a.L = L;
-    a.npairs = (N + L - 1) / (2 * L);
+  for (Level const& l : sched_) {
+    a.L = l.L;
+    a.npairs = l.npairs;
+    a.pairs = l.off >= 0 ? pairtab_.p + l.off : nullptr;
     timed(kForward, [&] {
       hipError_t e = hipSuccess;
       if (method_ == kLU && sweep_fast_ &&
@@ -350,7 +529,6 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
         hip_check(e, "forward_fast");
       else hip_check(launch_forward(method_, a, st_), "forward");
     });
-    L *= 2;
   }
   if (N > 1 && m > 0)
     timed(kReduce, [&] {
@@ -367,10 +545,10 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
     ls.U = svdU_.p; ls.V = svdV_.p; ls.S = svdS_.p; ls.rank = svd_rank_.p;
     hip_check(launch_last_solve<TX>(ls, lds_limit_, st_), "last solve");
   });
-  int k = Lmax_;
-  while ((k /= 2) > 0) {
-    a.L = k;
-    a.npairs = (N + k - 1) / (2 * k);
+  for (auto it = sched_.rbegin(); it != sched_.rend(); ++it) {
+    a.L = it->L;
+    a.npairs = it->npairs;
+    a.pairs = it->off >= 0 ? pairtab_.p + it->off : nullptr;
     timed(kBackward, [&] {
       hipError_t e = hipSuccess;
       if (method_ == kLU && sweep_fast_ &&
@@ -383,6 +561,7 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
 
 template <typename TA, typename TW>
 void Engine<TA, TW>::residual_async(double const* x, double const* b, double* r) {
+  if (!pD_) throw HipError("residual: no operator loaded or bound");
   MvArgs<TA> m{};
   m.N = d_.N; m.n = d_.n; m.qr = d_.qr; m.qx = d_.qx; m.nr = d_.nr; m.nx = d_.nx;
   m.D = pD_; m.E = pE_; m.B = pB_; m.C = pC_; m.H = pH_; m.Cq = pCq_; m.F = pF_;
@@ -394,6 +573,23 @@ void Engine<TA, TW>::residual_async(double const* x, double const* b, double* r)
 template <typename TA, typename TW>
 void Engine<TA, TW>::mv_async(double const* x, double* y) {
   residual_async(x, nullptr, y);
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::abs_bound_async(double const* x, double const* b, double* out) {
+  if (!pD_) throw HipError("abs_bound: no operator loaded or bound");
+  MvArgs<TA> m{};
+  m.N = d_.N; m.n = d_.n; m.qr = d_.qr; m.qx = d_.qx; m.nr = d_.nr; m.nx = d_.nx;
+  m.D = pD_; m.E = pE_; m.B = pB_; m.C = pC_; m.H = pH_; m.Cq = pCq_; m.F = pF_;
+  m.x = x; m.b = b; m.out = out; m.Cpart = Cpart_.p;
+  tmpd_.resize(std::max<size_t>(tmpd_.count, size_t(max_chunks_) * std::max(1, d_.nr)));
+  timed(kResidual, [&] { hip_check(launch_mv(m, Csum_.p, tmpd_.p, max_chunks_, st_, true), "abs mv"); });
+}
+
+template <typename TA, typename TW>
+void Engine<TA, TW>::max_ratio_async(double const* r, double const* den, int64_t len, double* out) {
+  tmpd_.resize(std::max<size_t>(tmpd_.count, size_t(max_chunks_)));
+  hip_check(launch_max_ratio(r, den, len, tmpd_.p, max_chunks_, out, st_), "max ratio");
 }
 
 template <typename TA, typename TW>

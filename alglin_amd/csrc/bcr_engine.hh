@@ -41,12 +41,15 @@ class HipError : public std::runtime_error {
 
 void hip_check(hipError_t e, char const* what);
 
-// A device buffer owned by the engine.
+// A device buffer owned by the engine, or a slice of a region the engine
+// does not own (borrow): then resize never allocates, it only checks.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t count = 0;
+  bool borrowed = false;
   void resize(size_t c);
+  void borrow(T* ptr, size_t c);
   void release();
   ~DevBuf() { release(); }
 };
@@ -60,12 +63,31 @@ template <typename TA, typename TW>
 class Engine {
  public:
   Engine();
+  // on the caller's stream (not owned, not created)
+  explicit Engine(hipStream_t external);
   ~Engine();
   Engine(Engine const&) = delete;
   Engine& operator=(Engine const&) = delete;
 
   void allocate(Dims const& d);
+  // as allocate() for pure cyclic reduction with LU eliminations and an LU
+  // last block, solving up to nrhs columns, every working array a slice of
+  // [base, base + bytes) (256-byte aligned): no device allocation, now or in
+  // the factorize / solve that follow.  Throws when the slices do not fit.
+  void allocate_in(Dims const& d, int nrhs, char* base, size_t bytes);
+  // bytes allocate_in needs for these dimensions
+  static size_t bytes_in(Dims const& d, int nrhs, int max_chunks);
+  int max_chunks() const { return max_chunks_; }
   Dims const& dims() const { return d_; }
+  // elimination order of BorderedCR with num_parallel_block = P
+  // (BABD_BorderedCR.cc:139-158): the blocks are split into P ranges, each
+  // range is reduced by cyclic reduction on its own (cc:1055-1171), then the
+  // chain of the 2P-1 range-end rows (cc:1175-1266); P = 1 (the default) is
+  // pure cyclic reduction over all blocks.  P is shrunk like the reference's
+  // until 2P <= N.
+  void set_partition(int P);
+  int partition() const { return used_P_; }
+  std::vector<int32_t> const& partition_blocks() const { return iblock_; }
   hipStream_t stream() const { return st_; }
   // run on another engine's stream (not owned) so that the two engines of one
   // solver object order their work on a single stream
@@ -103,6 +125,13 @@ class Engine {
   void residual_async(double const* x, double const* b, double* r);
   // y = A x
   void mv_async(double const* x, double* y);
+  // out = |A| |x| + |b| (FP64, pristine operator): the row scales of the
+  // componentwise backward error
+  void abs_bound_async(double const* x, double const* b, double* out);
+  // *out = max_i |r_i| / den_i over i < len (device scalar)
+  void max_ratio_async(double const* r, double const* den, int64_t len, double* out);
+  // true once an operator is uploaded or bound
+  bool has_operator() const { return pD_ != nullptr; }
   // ||v||_2 of a device vector into *out (device)
   void nrm2_async(double const* v, int64_t len, double* out);
 
@@ -132,7 +161,20 @@ class Engine {
   double acc_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t get_event();
 
-  void level_factor(int L, bool first);
+  // one elimination launch: stride L (pure cyclic reduction) or a pair
+  // table slice [off, off + 3 npairs) of pairtab_ (off >= 0)
+  struct Level {
+    int L = 1, npairs = 0, slot = 0;
+    bool first = false;
+    int64_t off = -1;
+  };
+  void build_schedule();
+  void level_factor(Level const& l);
+  std::vector<Level> sched_;
+  std::vector<int32_t> copy_rows_, copy_cols_;   // untouched by the first launch: copied from A
+  std::vector<int32_t> iblock_;
+  DevBuf<int32_t> pairtab_;
+  int want_P_ = 1, used_P_ = 1;
   Dims d_;
   int method_ = kLU;
   int last_method_ = kLastLU;
@@ -144,7 +186,6 @@ class Engine {
   size_t lds_limit_ = 65536;
   int max_chunks_ = 1024;
   FactorStats stats_;
-  int Lmax_ = 1;
   // pristine (owned copy or bound)
   DevBuf<TA> aD_, aE_, aB_, aC_, aH_, aCq_, aF_;
   TA const *pD_ = nullptr, *pE_ = nullptr, *pB_ = nullptr, *pC_ = nullptr, *pH_ = nullptr, *pCq_ = nullptr,

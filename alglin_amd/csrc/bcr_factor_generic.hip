@@ -70,7 +70,7 @@ __device__ __forceinline__ void pair_generic_body(FactorArgs<TS, TW> a, TW* __re
   const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
-  pair_index(p, a.L, a.N, j, jp, jpp);
+  pair_of(a, p, j, jp, jpp);
   const int n = a.n, nr = a.nr, nx = a.nx;
   const int n2 = 2 * n;
   const int R = n2 + nr;
@@ -133,8 +133,10 @@ __device__ __forceinline__ void pair_generic_body(FactorArgs<TS, TW> a, TW* __re
   for (int c = 0; c < n; ++c) {
     if (METHOD == kLU) {
       if (wid == 0) {
+        // a NaN never compares greater: the default pivot is the diagonal, so
+        // the index stays valid whatever the panel holds
         TW best = TW(-1);
-        int bi = 0x7fffffff;
+        int bi = c;
         for (int r = c + lane; r < n2; r += kWave) {
           TW v = dev_abs(P[r + c * ld]);
           if (v > best) { best = v; bi = r; }
@@ -169,7 +171,7 @@ __device__ __forceinline__ void pair_generic_body(FactorArgs<TS, TW> a, TW* __re
       if (METHOD == kQRP) {
         if (wid == 0) {
           TW best = TW(-1);
-          int bi = 0x7fffffff;
+          int bi = c;   // valid also when every norm is NaN
           for (int s = c + lane; s < n; s += kWave) {
             if (s_vn1[s] > best) { best = s_vn1[s]; bi = s; }
           }

@@ -73,7 +73,7 @@ __device__ __forceinline__ void lu_last_body(double* __restrict__ Hm, int Nc, in
   for (int c = 0; c < Nc; ++c) {
     if (wid == 0) {
       double best = -1;
-      int bi = 0x7fffffff;
+      int bi = c;   // a NaN column keeps the diagonal: the index stays valid
       for (int r = c + lane; r < Nc; r += 64) {
         const double v = fabs(A[r + c * ld]);
         if (v > best) { best = v; bi = r; }

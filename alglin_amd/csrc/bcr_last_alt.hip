@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(1024) lupq_last_kernel(double* __restrict__ Hm
   for (int k = 0; k < n; ++k) {
     const int m = n - k;
     double best = -1.0;
-    int bkey = 0x7fffffff;
+    int bkey = 0;   // valid also when every entry is NaN
     for (int t = tid; t < m * m; t += nth) {   // column-major order over the trailing block
       const double v = fabs(A[(k + t % m) + size_t(k + t / m) * n]);
       if (v > best) { best = v; bkey = t; }

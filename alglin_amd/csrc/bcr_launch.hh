@@ -58,8 +58,13 @@ hipError_t launch_svd_last(SvdArgs const& a, hipStream_t st);
 hipError_t launch_qr_last(bool pivot, double* Hm, int Nr, int Nc, double* tau, int32_t* jpvt, int32_t* status,
                           size_t lds_limit, hipStream_t st);
 
+// r = b - A x, y = A x, or (absval) |A| |x| + |b|
 template <typename TA>
-hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st);
+hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st,
+                     bool absval = false);
+// *out = max_i |r_i| / den_i (device scalar)
+hipError_t launch_max_ratio(const double* r, const double* den, int64_t len, double* tmp, int max_chunks, double* out,
+                            hipStream_t st);
 hipError_t launch_nrm2(const double* v, int64_t len, double* tmp, int max_chunks, double* out, hipStream_t st);
 
 // dst[i] = TD(src[i]) for i < count  (also a zero-fill when src == nullptr)

@@ -8,6 +8,11 @@
 //                  (sum of the partials + Cq q + F xb)
 // The per-block partials make the border rows a deterministic reduction
 // instead of N serial gemv accumulations.
+//
+// ABS = true computes |A| |x| (+ |b| when b is given) with the same loops:
+// the denominator of the componentwise backward error
+// max_i |b - A x|_i / (|A| |x| + |b|)_i (Oettli-Prager) that the
+// refinement uses to recognise the FP64 residual floor.
 #include "bcr_launch.hh"
 #include "bcr_wave.hh"
 
@@ -21,7 +26,10 @@ namespace bcr {
 // sub-row * g + row), so small blocks do not leave most lanes idle; x is held one entry per lane and broadcast per
 // column (v_readlane with one block row per wave, ds_bpermute within a
 // sub-row group otherwise).
-template <typename TA, bool PACKED>
+template <bool ABS>
+__device__ __forceinline__ double mag(double v) { return ABS ? fabs(v) : v; }
+
+template <typename TA, bool PACKED, bool ABS>
 __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
   constexpr int CH = 8;
   const int lane = threadIdx.x & 63;
@@ -41,7 +49,7 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
   };
   const double* xk = a.x + size_t(kc) * n;
   const double* xb = a.x + size_t(N + 1) * n + a.qx;
-  const double xa = r < n ? xk[r < n ? r : 0] : 0.0;   // x_k, one entry per lane
+  const double xa = r < n ? mag<ABS>(xk[r < n ? r : 0]) : 0.0;   // x_k, one entry per lane
   if (k0 < N) {
     const bool row = live && k < N && r < n;
     const int kr = row ? k : k0;
@@ -52,7 +60,7 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
     const TA* Bk = a.B + size_t(k0) * n * nx;
     const int rr = r < n ? r : 0;
     const unsigned odk = unsigned(kr - k0) * unsigned(n * n) + rr, obk = unsigned(kr - k0) * unsigned(n * nx) + rr;
-    const double xn = (row ? xk : a.x + size_t(k0) * n)[n + rr];   // x_{k+1}
+    const double xn = mag<ABS>((row ? xk : a.x + size_t(k0) * n)[n + rr]);   // x_{k+1}
     // the border partial C_k x_k of the same columns rides along (lanes
     // (sub-row, i) with i < nr; a sub-row k = N has only this part)
     const bool crow = live && r < nr;
@@ -67,7 +75,7 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
     for (int i = 0; i < CB; ++i) {
       const int cc = i < nx ? i : 0;
       bb0[i] = nx > 0 ? gload(Bk, obk + unsigned(cc * n)) : TA(0);
-      w0[i] = nx > 0 ? xb[cc] : 0.0;
+      w0[i] = nx > 0 ? mag<ABS>(xb[cc]) : 0.0;
     }
     const size_t out = size_t(kr) * n + rr;
     const double bval = a.b ? a.b[out] : 0.0;
@@ -85,13 +93,13 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
       for (int i = 0; i < CH; ++i) {
         const int cc = c0 + i < n ? c0 + i : 0;
         const double u = bcast(xa, cc), v = bcast(xn, cc);
-        acc += keep_if(double(d[i]) * u + double(e[i]) * v, c0 + i < n);
-        accc += keep_if(double(cv[i]) * u, c0 + i < n);
+        acc += keep_if(mag<ABS>(double(d[i])) * u + mag<ABS>(double(e[i])) * v, c0 + i < n);
+        accc += keep_if(mag<ABS>(double(cv[i])) * u, c0 + i < n);
       }
     }
     if (nr > 0 && crow) a.Cpart[size_t(k) * nr + r] = accc;
 #pragma unroll
-    for (int i = 0; i < CB; ++i) acc += keep_if(double(bb0[i]) * w0[i], i < nx);
+    for (int i = 0; i < CB; ++i) acc += keep_if(mag<ABS>(double(bb0[i])) * w0[i], i < nx);
     for (int c0 = CB; c0 < nx; c0 += CH) {
       TA bb[CH];
       double w[CH];
@@ -99,12 +107,12 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
       for (int i = 0; i < CH; ++i) {
         const int cc = c0 + i < nx ? c0 + i : 0;
         bb[i] = gload(Bk, obk + unsigned(cc * n));
-        w[i] = xb[cc];
+        w[i] = mag<ABS>(xb[cc]);
       }
 #pragma unroll
-      for (int i = 0; i < CH; ++i) acc += keep_if(double(bb[i]) * w[i], c0 + i < nx);
+      for (int i = 0; i < CH; ++i) acc += keep_if(mag<ABS>(double(bb[i])) * w[i], c0 + i < nx);
     }
-    if (row) a.out[out] = a.b ? bval - acc : acc;
+    if (row) a.out[out] = ABS ? acc + fabs(bval) : (a.b ? bval - acc : acc);
   }
   // border partials C_k x_k of a wave that starts at block row N (the
   // others formed them above): lanes (sub-row, i) with i < nr
@@ -125,26 +133,26 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
 #pragma unroll
       for (int t = 0; t < CH; ++t) {
         const int cc = c0 + t < n ? c0 + t : 0;
-        acc += keep_if(double(cv[t]) * bcast(xa, cc), c0 + t < n);
+        acc += keep_if(mag<ABS>(double(cv[t])) * bcast(xa, cc), c0 + t < n);
       }
     }
     if (crow) a.Cpart[size_t(k) * nr + i] = acc;
   }
 }
 
-template <typename TA>
+template <typename TA, bool ABS = false>
 __device__ __forceinline__ void mv_rows_body(MvArgs<TA> const& a, int bid) {
-  if (mv_rows_per_wave(a.n, a.nr) > 1) mv_rows_run<TA, true>(a, bid);
-  else mv_rows_run<TA, false>(a, bid);
+  if (mv_rows_per_wave(a.n, a.nr) > 1) mv_rows_run<TA, true, ABS>(a, bid);
+  else mv_rows_run<TA, false, ABS>(a, bid);
 }
 
-template <typename TA>
+template <typename TA, bool ABS>
 __global__ void __launch_bounds__(256) mv_rows_kernel(MvArgs<TA> a) {
-  mv_rows_body<TA>(a, blockIdx.x);
+  mv_rows_body<TA, ABS>(a, blockIdx.x);
 }
 
 
-template <typename TA>
+template <typename TA, bool ABS = false>
 __device__ __forceinline__ void mv_tail_body(MvArgs<TA> const& a, const double* __restrict__ Csum) {
   const int N = a.N, n = a.n, qr = a.qr, qx = a.qx, nr = a.nr, nx = a.nx;
   const int m = n + qr;
@@ -163,23 +171,23 @@ __device__ __forceinline__ void mv_tail_body(MvArgs<TA> const& a, const double* 
         else if (c < 2 * n) xv = xN[c - n];
         else if (c < 2 * n + qx) xv = xq[c - 2 * n];
         else xv = xb[c - 2 * n - qx];
-        acc += double(a.H[i + size_t(c) * m]) * xv;
+        acc += mag<ABS>(double(a.H[i + size_t(c) * m])) * mag<ABS>(xv);
       }
       row = size_t(N) * n + i;
     } else {
       const int r = i - m;
       acc = Csum[r];
-      for (int c = 0; c < qx; ++c) acc += double(a.Cq[r + c * nr]) * xq[c];
-      for (int c = 0; c < nx; ++c) acc += double(a.F[r + c * nr]) * xb[c];
+      for (int c = 0; c < qx; ++c) acc += mag<ABS>(double(a.Cq[r + c * nr])) * mag<ABS>(xq[c]);
+      for (int c = 0; c < nx; ++c) acc += mag<ABS>(double(a.F[r + c * nr])) * mag<ABS>(xb[c]);
       row = size_t(N + 1) * n + qr + r;
     }
-    a.out[row] = a.b ? a.b[row] - acc : acc;
+    a.out[row] = ABS ? acc + (a.b ? fabs(a.b[row]) : 0.0) : (a.b ? a.b[row] - acc : acc);
   }
 }
 
-template <typename TA>
+template <typename TA, bool ABS>
 __global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double* __restrict__ Csum) {
-  mv_tail_body<TA>(a, Csum);
+  mv_tail_body<TA, ABS>(a, Csum);
 }
 
 template <typename TA>
@@ -242,14 +250,69 @@ __global__ void __launch_bounds__(256) sumsq_final_kernel(const double* __restri
 }
 
 template <typename TA>
-hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st) {
+hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st, bool absval) {
   const int rows_per_block = 4 * mv_rows_per_wave(a.n, a.nr);
-  hipLaunchKernelGGL((mv_rows_kernel<TA>), dim3((a.N + 1 + rows_per_block - 1) / rows_per_block), dim3(256), 0, st, a);
+  const dim3 grid((a.N + 1 + rows_per_block - 1) / rows_per_block);
+  if (absval) hipLaunchKernelGGL((mv_rows_kernel<TA, true>), grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((mv_rows_kernel<TA, false>), grid, dim3(256), 0, st, a);
   if (a.nr > 0) {
     launch_convert<double, double>(nullptr, Csum, a.nr, st);
     launch_reduce<double, double>(a.Cpart, 0, a.N + 1, a.nr, tmp, max_chunks, Csum, a.nr, a.nr, st);
   }
-  hipLaunchKernelGGL((mv_tail_kernel<TA>), dim3(1), dim3(256), 0, st, a, Csum);
+  if (absval) hipLaunchKernelGGL((mv_tail_kernel<TA, true>), dim3(1), dim3(256), 0, st, a, Csum);
+  else hipLaunchKernelGGL((mv_tail_kernel<TA, false>), dim3(1), dim3(256), 0, st, a, Csum);
+  return hipGetLastError();
+}
+
+// max_i |r_i| / den_i over i < len (0 / 0 counts as 0, r / 0 as +inf):
+// pass 1 per chunk into tmp, pass 2 into *out
+__global__ void __launch_bounds__(256) max_ratio_partial_kernel(const double* __restrict__ r,
+                                                                const double* __restrict__ den, int64_t len,
+                                                                double* __restrict__ tmp) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += int64_t(gridDim.x) * blockDim.x) {
+    const double ar = fabs(r[i]), d = den[i];
+    const double q = ar == 0.0 ? 0.0 : (d > 0.0 ? ar / d : INFINITY);
+    s = q > s || q != q ? q : s;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double o = red[threadIdx.x + w];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tmp[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) max_final_kernel(const double* __restrict__ tmp, int nb,
+                                                        double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s = tmp[i] > s || tmp[i] != tmp[i] ? tmp[i] : s;
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double o = red[threadIdx.x + w];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+hipError_t launch_max_ratio(const double* r, const double* den, int64_t len, double* tmp, int max_chunks, double* out,
+                            hipStream_t st) {
+  int64_t nb64 = (len + 255) / 256;
+  if (nb64 < 1) nb64 = 1;
+  if (nb64 > max_chunks) nb64 = max_chunks;
+  const int nb = int(nb64);
+  hipLaunchKernelGGL(max_ratio_partial_kernel, dim3(nb), dim3(256), 0, st, r, den, len, tmp);
+  hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, st, tmp, nb, out);
   return hipGetLastError();
 }
 
@@ -263,7 +326,7 @@ hipError_t launch_nrm2(const double* v, int64_t len, double* tmp, int max_chunks
   return hipGetLastError();
 }
 
-template hipError_t launch_mv<double>(MvArgs<double> const&, double*, double*, int, hipStream_t);
-template hipError_t launch_mv<float>(MvArgs<float> const&, double*, double*, int, hipStream_t);
+template hipError_t launch_mv<double>(MvArgs<double> const&, double*, double*, int, hipStream_t, bool);
+template hipError_t launch_mv<float>(MvArgs<float> const&, double*, double*, int, hipStream_t, bool);
 
 }  // namespace bcr

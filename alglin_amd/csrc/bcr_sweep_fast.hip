@@ -52,7 +52,7 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
   const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
   const int64_t ldx = a.ldx;
   int j = 0, jp = 0, jpp = 0;
-  if (live) pair_index(p, a.L, a.N, j, jp, jpp);
+  if (live) pair_of(a, p, j, jp, jpp);
   const bool row = live && r < n2;
 
   // row r of the 2n x n panel [L; M] (column-major, ld 2n)
@@ -150,7 +150,7 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
   const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
   const int64_t ldx = a.ldx;
   int j = 0, jp = 0, jpp = 0;
-  if (live) pair_index(p, a.L, a.N, j, jp, jpp);
+  if (live) pair_of(a, p, j, jp, jpp);
   const int h = r / NMAX, rr = r % NMAX;
   const bool row = live && rr < n;
   const size_t nn = size_t(n) * n;

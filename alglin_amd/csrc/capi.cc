@@ -6,10 +6,11 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <type_traits>
 
-#include "BorderedCR.hh"
+#include "bcr_gpu.hh"
 #include "bcr_batch.hh"
 #include "bcr_mlp.hh"
 
@@ -42,12 +43,8 @@ auto on(alglin_bcr_t h, Fn&& fn) {
 }
 
 int on_(alglin_bcr_t h, int on) {
-  if (h->dtype == 0) {
-    h->d->engine().set_timing(on != 0);
-    if (h->d->mixed_engine()) h->d->mixed_engine()->engine().set_timing(on != 0);
-  } else {
-    h->f->engine().set_timing(on != 0);
-  }
+  if (h->dtype == 0) h->d->gpu().set_timing(on != 0);
+  else h->f->gpu().set_timing(on != 0);
   return 0;
 }
 
@@ -231,7 +228,7 @@ int alglin_bcr_sparse_load(alglin_bcr_t h, void const* V, int32_t const* I, int3
     return on(h, [&](auto& s, auto* tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
       s.sparse_load(static_cast<T const*>(V), I, r_offs, J, c_offs, nnz, do_error != 0);
-      return do_error ? 0 : (s.last_error() == "no error" ? 0 : 0);
+      return 0;
     });
   });
 }
@@ -243,13 +240,81 @@ int alglin_bcr_check_matrix(alglin_bcr_t h) {
 char const* alglin_bcr_last_error(alglin_bcr_t h) {
   if (!h) return "null handle";
   if (!h->err.empty()) return h->err.c_str();
-  return on(h, [&](auto& s, auto*) { return s.last_error().c_str(); });
+  return on(h, [&](auto& s, auto*) { return s.last_error().data(); });
 }
 
 char const* alglin_bcr_info(alglin_bcr_t h) {
   if (!h) return "";
   h->info = on(h, [&](auto& s, auto*) { return s.info(); });
   return h->info.c_str();
+}
+
+int alglin_bcr_set_debug(alglin_bcr_t h, int flag) {
+  return guarded(h, [&] { return on(h, [&](auto& s, auto*) { s.set_debug(flag != 0); return 0; }); });
+}
+
+int alglin_bcr_set_num_parallel_block(alglin_bcr_t h, int32_t npb) {
+  return guarded(h, [&] { return on(h, [&](auto& s, auto*) { s.set_num_parallel_block(npb); return 0; }); });
+}
+
+int32_t alglin_bcr_used_parallel_block(alglin_bcr_t h) {
+  if (!h) return -1;
+  return on(h, [&](auto& s, auto*) { return s.used_parallel_block(); });
+}
+
+int alglin_bcr_dup(alglin_bcr_t h, alglin_bcr_t src) {
+  return guarded(h, [&] {
+    if (!src || src->dtype != h->dtype) throw std::invalid_argument("alglin_bcr_dup: source of another type");
+    if (h->dtype == 0) h->d->dup(*src->d);
+    else h->f->dup(*src->f);
+    h->bound = false;
+    return 0;
+  });
+}
+
+int alglin_bcr_t_solve(alglin_bcr_t h, void* x) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      return s.t_solve(static_cast<T*>(x)) ? 0 : 1;
+    });
+  });
+}
+
+int64_t alglin_bcr_matlab_script(alglin_bcr_t h, char* buf, int64_t len) {
+  if (!h) return -1;
+  try {
+    std::ostringstream o;
+    on(h, [&](auto& s, auto*) { s.print_matlab_script(o); return 0; });
+    std::string const t = o.str();
+    if (buf && len > 0) {
+      const size_t k = std::min<size_t>(t.size(), size_t(len - 1));
+      std::memcpy(buf, t.data(), k);
+      buf[k] = 0;
+    }
+    return int64_t(t.size()) + 1;
+  } catch (std::exception const& e) {
+    h->err = e.what();
+    return -1;
+  }
+}
+
+int alglin_bcr_get_packed(alglin_bcr_t h, void* D, void* E, void* B, void* C, void* H, void* Cq, void* F) {
+  return guarded(h, [&] {
+    return on(h, [&](auto& s, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      const size_t N = s.number_of_blocks(), n = s.block_size(), nr = s.dim_nr(), nx = s.dim_nx();
+      auto cp = [](void* dst, T const* src, size_t cnt) { if (dst && cnt) std::memcpy(dst, src, cnt * sizeof(T)); };
+      cp(D, s.data_D(), N * n * n);
+      cp(E, s.data_E(), N * n * n);
+      cp(B, s.data_B(), N * n * nx);
+      cp(C, s.data_C(), (N + 1) * nr * n);
+      cp(H, s.data_H(), (n + s.dim_qr()) * size_t(s.Nc()));
+      cp(Cq, s.data_Cq(), nr * s.dim_qx());
+      cp(F, s.data_F(), nr * nx);
+      return 0;
+    });
+  });
 }
 
 int alglin_bcr_bind_device(alglin_bcr_t h, void const* D, void const* E, void const* B, void const* C, void const* H,
@@ -283,7 +348,7 @@ int alglin_bcr_solve_device(alglin_bcr_t h, void* X, int32_t nrhs, int64_t ld, i
       using T = std::remove_pointer_t<decltype(tag)>;
       s.solve_device(static_cast<T*>(X), nrhs, ld, sync != 0);
       const int rs = s.refine_report().status;
-      if (s.mixed() && rs != 0 && rs != 4) {   // 4: solved by the FP64 QRP fallback
+      if (s.mixed() && !bcr::refine_ok(rs)) {   // 0 converged, 4 FP64 QRP fallback, 5 FP64 floor
         h->err = "iterative refinement did not reach the requested residual";
         return 1;
       }
@@ -301,9 +366,7 @@ int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32
 int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) {
-      s.engine().set_pair_variant(variant);
-      if constexpr (std::is_same_v<std::remove_reference_t<decltype(s)>, alglin_amd::BorderedCR<double>>)
-        if (s.mixed_engine()) s.mixed_engine()->engine().set_pair_variant(variant);
+      s.gpu().set_pair_variant(variant);
       return 0;
     });
   });
@@ -315,11 +378,12 @@ int alglin_bcr_set_fallback(alglin_bcr_t h, int on_) {
   });
 }
 
-int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]) {
+int alglin_bcr_refine_report(alglin_bcr_t h, double out[5]) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) {
-      auto const& r = s.refine_report();
+      auto const r = s.refine_report();
       out[0] = r.iterations; out[1] = r.rel_residual; out[2] = r.first_residual; out[3] = r.status;
+      out[4] = r.backward_error;
       return 0;
     });
   });
@@ -328,9 +392,7 @@ int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]) {
 int alglin_bcr_residual_device(alglin_bcr_t h, double const* x, double const* b, double* r, int sync) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) {
-      auto& e = s.engine();
-      e.residual_async(x, b, r);
-      if (sync) bcr::hip_check(hipStreamSynchronize(e.stream()), "sync");
+      s.residual_device(x, b, r, sync != 0);
       return 0;
     });
   });
@@ -339,7 +401,7 @@ int alglin_bcr_residual_device(alglin_bcr_t h, double const* x, double const* b,
 int alglin_bcr_sync(alglin_bcr_t h) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) {
-      bcr::hip_check(hipStreamSynchronize(s.engine().stream()), "sync");
+      s.synchronize();
       return 0;
     });
   });
@@ -347,7 +409,12 @@ int alglin_bcr_sync(alglin_bcr_t h) {
 
 void* alglin_bcr_stream(alglin_bcr_t h) {
   if (!h) return nullptr;
-  return on(h, [&](auto& s, auto*) { return static_cast<void*>(s.engine().stream()); });
+  try {
+    return on(h, [&](auto& s, auto*) { return s.stream(); });
+  } catch (std::exception const& e) {
+    h->err = e.what();
+    return nullptr;
+  }
 }
 
 int alglin_bcr_get_factors(alglin_bcr_t h, void* T_, int32_t* perm, void* D, void* E, void* B, void* C,
@@ -355,7 +422,7 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T_, int32_t* perm, void* D, voi
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto* tag) {
       using T = std::remove_pointer_t<decltype(tag)>;
-      auto& e = s.engine();
+      auto& e = s.gpu().fp();
       const size_t N = s.number_of_blocks(), n = s.block_size(), nr = s.dim_nr(), nx = s.dim_nx();
       auto cp = [&](void* dst, void const* src, size_t bytes) {
         if (dst && bytes) bcr::hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H");
@@ -381,7 +448,10 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T_, int32_t* perm, void* D, voi
 
 int alglin_bcr_fast_path(alglin_bcr_t h) {
   if (!h) return 0;
-  return on(h, [&](auto& s, auto*) { return s.engine().stats().fast_path; });
+  return on(h, [&](auto& s, auto*) {
+    auto& g = s.gpu();
+    return g.eng ? g.eng->stats().fast_path : 0;
+  });
 }
 
 int alglin_bcr_set_timing(alglin_bcr_t h, int on) {
@@ -391,13 +461,13 @@ int alglin_bcr_set_timing(alglin_bcr_t h, int on) {
 int alglin_bcr_timing(alglin_bcr_t h, double out[8], int reset) {
   return guarded(h, [&] {
     return on(h, [&](auto& s, auto*) {
-      s.engine().timing(out, reset != 0);
-      if constexpr (std::is_same_v<std::remove_reference_t<decltype(s)>, BorderedCR<double>>) {
-        if (s.mixed_engine()) {   // phases of the mixed engine add to those of the FP64 one
-          double m[8];
-          s.mixed_engine()->engine().timing(m, reset != 0);
-          for (int i = 0; i < 8; ++i) out[i] += m[i];
-        }
+      auto& g = s.gpu();
+      for (int i = 0; i < 8; ++i) out[i] = 0;
+      if (g.eng) g.eng->timing(out, reset != 0);
+      if (g.mix) {   // phases of the mixed engine add to those of the FP64 one
+        double m[8];
+        g.mix->engine().timing(m, reset != 0);
+        for (int i = 0; i < 8; ++i) out[i] += m[i];
       }
       return 0;
     });

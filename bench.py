@@ -3,14 +3,21 @@
 Default workload (--workload batched): 4096 uneven BABD systems in one
 batched solve, n uniform in 4..48, N log-uniform in [16, 4096],
 qr = qx = n/4, nr = nx = 2, Gaussian blocks with diagonal shift 4, generated
-on the GPU, random right-hand sides.  One step = the whole batch solved to a
-relative residual of 1e-12 per system: FP32 cyclic-reduction factors, FP64
-iterative refinement, the factor/solve, residual and correction launch lists
-replayed as HIP graphs.  value = systems solved per second.  Reported beside
-it: the same batch in plain FP64 through the same batched engine, and
-through the plain FP64 BorderedCR one system at a time (the reference's call
-sequence, timed on a sample and scaled); the per-system sweep counts; a CPU
-baseline (the oracle on a sample of the systems, one system per host thread).
+on the GPU, random right-hand sides.  One step = the whole batch through the
+mixed + learned path: FP32 cyclic-reduction factors, the learned LU/QRP
+selector on the first solve's features, FP64 iterative refinement to a
+relative residual of 1e-12 per system, the FP64 fallback (LU, then QRP) for
+the systems that miss it; the factor/solve, residual and correction launch
+lists replayed as HIP graphs.  value = systems that reach 1e-12, per second.
+Reported beside it: the same batch in plain FP64 through the same batched
+engine and through the plain FP64 BorderedCR one system at a time (the
+reference's call sequence, timed on a sample and scaled); the mixed solve
+without the selector; per-system sweep counts; the forward error against the
+FP64 solutions; a CPU baseline (the oracle on a sample of the systems, one
+system per host thread); and three other workloads measured in the same run:
+the large single FP64 system (N = 100000, n = 32), the n = 64 mixed system
+with kappa(D_k) up to 1e6 and 4 right-hand sides, and the 8192-system
+selector workload.
 
 Other workloads:
   --workload selector  8192 systems, half nearly singular, the learned LU/QRP
@@ -21,7 +28,7 @@ Other workloads:
                        --mode mixed, with the pair-elimination roofline and
                        the 16-RHS time.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload batched]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-subs] [--workload batched]
 """
 from __future__ import annotations
 
@@ -62,6 +69,31 @@ def cpu_threads():
         return len(os.sched_getaffinity(0))
     except AttributeError:
         return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of the dominant class as the committed rocprof
+    FETCH_SIZE / WRITE_SIZE summary states it (a line
+    `traffic_bytes_per_launch = <bytes>` in profiles/<name>), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def run_cpu_baseline(N, n, qr, qx, nr, nx, shift, seed, sample_N, reps=3):
@@ -116,10 +148,11 @@ def batch_sweep_bytes(dims, s):
     return f, r
 
 
-def run_cpu_baseline_batch(bt, dims, sample, reps=2):
+def run_cpu_baseline_batch(bt, dims, sample, reps=5):
     """The oracle on `sample` systems of the batch, one system per host thread
     (num_parallel_block = 1 each, as the reference runs independent systems),
-    scaled to the whole batch by the N n^3 work ratio."""
+    one warm-up then the median of `reps` runs, scaled to the whole batch by
+    the N n^3 work ratio."""
     from concurrent.futures import ThreadPoolExecutor
     from alglin_amd import systems
     from oracle import pyoracle
@@ -148,10 +181,11 @@ def run_cpu_baseline_batch(bt, dims, sample, reps=2):
     t = float(np.median(times))
     w = (dims[:, 0].astype(float) * dims[:, 1].astype(float) ** 3)
     scale = w.sum() / w[:sample].sum()
-    return dict(value=len(dims) / (t * scale), unit="solves/s", cores=P, kind="port",
-                sample=f"oracle C++ restatement, {sample} systems of the batch on {P} threads (one system per "
-                       f"thread, FP64 LU), median of {reps} runs = {t:.3f} s, scaled x{scale:.1f} by N*n^3 work",
-                seconds_sample=t)
+    return dict(value=len(dims) / (t * scale), unit="solves/s", cores=P, kind="port", cpu=cpu_model(),
+                sample=f"oracle C++ restatement, {sample} systems of the batch on {P} threads of {cpu_model()} "
+                       f"(one system per thread, FP64 LU), one warm-up then the median of {reps} runs = {t:.3f} s, "
+                       f"scaled x{scale:.1f} by N*n^3 work",
+                seconds_sample=t, runs_s=times)
 
 
 def time_plain_per_system(bt, dims, B, sample, seed):
@@ -205,10 +239,25 @@ def time_plain_per_system(bt, dims, B, sample, seed):
                        f"{1e3 * t:.1f} ms, scaled x{scale:.1f}")
 
 
+def forward_errors(bt, dims, X, Xref):
+    """Per system ||x - x_ref|| / ||x_ref|| (2-norms over the system's
+    unknowns) of two device solution arrays in the batch's x layout."""
+    x, xr = X.to_host(), Xref.to_host()
+    out = np.zeros(len(dims))
+    for s, (N, n, qr, qx, nr, nx, _) in enumerate(dims):
+        o = bt.offset("x", s)
+        nc = int(n) * (int(N) + 1) + int(qx) + int(nx)
+        d = x[o:o + nc] - xr[o:o + nc]
+        out[s] = np.linalg.norm(d) / max(np.linalg.norm(xr[o:o + nc]), 1e-300)
+    return out
+
+
 def bench_batched(args):
-    """Batched uneven systems: plain FP64 cyclic reduction vs FP32 factors +
-    FP64 refinement (graph replay), or, for the selector workload, vs the
-    mixed solve with the learned LU/QRP choice and the FP64 fallback (LU, then QRP)."""
+    """Batched uneven systems: the mixed + learned path (FP32 factors, FP64
+    refinement, the LU/QRP selector with the FP64 fallback, launch lists
+    replayed as HIP graphs) against plain FP64 cyclic reduction in the same
+    batched engine; for the selector workload, also against the mixed solve
+    without the selector."""
     from alglin_amd import selector as S
     from alglin_amd import systems
     from alglin_amd.bcr import Batch, DeviceArray, device_info
@@ -238,6 +287,7 @@ def bench_batched(args):
         cb.free()
     B = DeviceArray.from_host(np.random.default_rng(args.seed + 5).standard_normal(L))
     X = DeviceArray(L)
+    X64 = DeviceArray(L)
     print(f"[bench] {nsys} systems generated on the GPU in {time.perf_counter() - t_gen:.1f} s "
           f"(operator {8 * (2 * ref.size('DE') + ref.size('B') + ref.size('C')) / 1e9:.1f} GB)",
           file=sys.stderr, flush=True)
@@ -251,26 +301,32 @@ def bench_batched(args):
         return (time.perf_counter() - t0) / steps
 
     tol = 1e-12
-    t64 = timed(lambda: ref.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
+    t64 = timed(lambda: ref.solve(B, X64, tol, 30, graph=True), args.steps, args.warmup)
     r64 = ref.report()
     try:
         per_sys = time_plain_per_system(mix, dims, B, args.per_system_sample, args.seed)
     except Exception as e:   # reported, not fatal
         per_sys = {"error": str(e)}
     del ref   # its FP64 factors; the operator stays alive through mix
+    mlp = S.load()
     out = {}
+    # the mixed solve without the selector (graph replay), for comparison
+    t_plain = timed(lambda: mix.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
+    r_plain = mix.report()
+    out["mixed_without_selector"] = dict(solves_per_s=(nsys - r_plain["failed"]) / t_plain, ms=1e3 * t_plain,
+                                         failed=r_plain["failed"], max_rel_residual=r_plain["max_rel_residual"],
+                                         max_sweeps=r_plain["max_sweeps"])
+    # the timed path: mixed + learned selector + FP64 fallback
+    tmix = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=True), args.steps, args.warmup)
+    rmix = mix.report()
+    chosen = mix.selection()
+    fl = chosen["chose_qrp"]
+    out["selector"] = dict(flagged=int(fl.sum()), fp64_path=int(chosen["fell_back"].sum()),
+                           fp64_path_unflagged=int((chosen["fell_back"] & ~fl).sum()),
+                           fp64_lu=int((chosen["fallback_stage"] == 1).sum()),
+                           fp64_qrp=int((chosen["fallback_stage"] == 2).sum()))
     if sel:
-        mlp = S.load()
-        t_none = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=None, fallback=True), args.steps,
-                       args.warmup)
-        r_none = mix.report()
-        fb_none = int(mix.selection()["fell_back"].sum())
-        tmix = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=True), args.steps, args.warmup)
-        rmix = mix.report()
-        chosen = mix.selection()
-        out["mixed_without_selector"] = dict(solves_per_s=nsys / t_none, ms=1e3 * t_none, fp64_fallbacks=fb_none,
-                                             max_sweeps=r_none["max_sweeps"],
-                                             max_rel_residual=r_none["max_rel_residual"])
+        out["selector"]["singular_systems"] = int((sing >= 1).sum())
         # without the FP64 safety net: how many systems miss the tolerance
         # with FP32 LU everywhere and with the selector's LU/QRP choice
         mix.solve_selected(B, X, tol, 30, selector=None, fallback=False)
@@ -278,22 +334,17 @@ def bench_batched(args):
         mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=False)
         f_sel = int(mix.report()["failed"])
         out["without_fallback"] = dict(failed_fp32_lu_only=f_lu, failed_with_selector=f_sel)
-        # with the fallback on, flagged systems go straight to the FP64 path
-        # (LU, then QRP); the rest join it only if FP32 LU + refinement fails
-        fl = chosen["chose_qrp"]
-        out["selector"] = dict(flagged=int(fl.sum()),
-                               fp64_path=int(chosen["fell_back"].sum()),
-                               fp64_path_unflagged=int((chosen["fell_back"] & ~fl).sum()),
-                               fp64_lu=int((chosen["fallback_stage"] == 1).sum()),
-                               fp64_qrp=int((chosen["fallback_stage"] == 2).sum()),
-                               singular_systems=int((sing >= 1).sum()))
-    else:
-        tmix = timed(lambda: mix.solve(B, X, tol, 30, graph=True), args.steps, args.warmup)
-        rmix = mix.report()
-    sweeps = int(rmix["max_sweeps"])
+        mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=True)   # X of the timed path again
+    ferr = forward_errors(mix, dims, X, X64)
+    solved = nsys - int(rmix["failed"])
+    sweeps = rmix["sweeps"].astype(float)
     fac = batch_pair_bytes(dims, 4)
-    sw, res = batch_sweep_bytes(dims, 4)
-    nbytes = fac + (1 + sweeps) * sw + (1 + sweeps) * res
+    # algorithmic bytes of the whole solve: factors once, then per system its
+    # own number of solve + residual passes (first solve + one per sweep)
+    nbytes = fac
+    for (N, n, qr, qx, nr, nx, m), k in zip(dims, sweeps):
+        f1, r1 = batch_sweep_bytes(np.array([[N, n, qr, qx, nr, nx, m]]), 4)
+        nbytes += (1.0 + k) * (f1 + r1)
     # dominant kernel: the pair-elimination launch class with the most time in
     # one eager factorization, every launch timed with HIP events on the
     # batch's stream
@@ -305,20 +356,21 @@ def bench_batched(args):
                  4: "pair_lu_t_batched + pair_rhs_mfma_batched (32 < n <= 48, FP32 factors; "
                     "pair_lu_cols_batched with --pair-kernel 4)", 5: "pair_generic_batched QR", 6: "pair_generic_batched QRP"}
     per_cls = {}
-    for lev, cls, nb, ms, by, fl in tf:
+    for lev, cls, nb, ms, by, flp in tf:
         c = per_cls.setdefault(int(cls), [0, 0.0, 0.0, 0.0])
         c[0] += 1
         c[1] += ms
         c[2] += by
-        c[3] += fl
+        c[3] += flp
     dom = max(per_cls, key=lambda k: per_cls[k][1])
     nl, ms_d, by_d, fl_d = per_cls[dom]
     achieved = (by_d / nl) / (ms_d / nl * 1e-3) / 1e9
-    print(f"[bench] plain FP64 {1e3 * t64:.1f} ms, mixed{'+learned' if sel else ''} {1e3 * tmix:.1f} ms, "
-          f"max residual {rmix['max_rel_residual']:.2e}", file=sys.stderr, flush=True)
+    traffic = profiled_traffic("batched_uneven_4096.md") if not sel else None
+    print(f"[bench] plain FP64 {1e3 * t64:.1f} ms, mixed {1e3 * t_plain:.1f} ms, mixed+learned {1e3 * tmix:.1f} ms, "
+          f"max residual {rmix['max_rel_residual']:.2e}, {rmix['failed']} above tol", file=sys.stderr, flush=True)
     result = {
         "metric": "batched BABD solves/sec @ <=1e-12 rel. residual, mixed+learned vs plain FP64",
-        "value": nsys / tmix,
+        "value": solved / tmix,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -333,18 +385,27 @@ def bench_batched(args):
             "workload": (f"{nsys} uneven BABD systems, n in {{8,16,32}}, N in [64,2048], half with a rank-deficient "
                          f"pair column (1e-10), mixed + learned LU/QRP selector + FP64 fallback (LU, then QRP)" if sel else
                          f"{nsys} uneven BABD systems, n in 4..48, N log-uniform [16,4096], qr=qx=n/4, nr=nx=2, "
-                         f"shift 4, FP32 factors + FP64 refinement, graph replay"),
+                         f"shift 4, FP32 factors + FP64 refinement + learned LU/QRP selector with the FP64 "
+                         f"fallback, graph replay"),
             "nsys": nsys, "tol": tol, "parallelism": "single-gpu",
         },
-        "plain_fp64": dict(solves_per_s=nsys / t64, ms=1e3 * t64, max_rel_residual=r64["max_rel_residual"],
-                           failed=r64["failed"], sweeps_hist=np.bincount(r64["sweeps"]).tolist()),
+        "solved_systems": solved,
+        "value_counts": "systems whose FP64 relative residual ||b - A x|| / ||b|| is <= 1e-12 (failed ones excluded)",
+        "plain_fp64": dict(solves_per_s=(nsys - r64["failed"]) / t64, ms=1e3 * t64,
+                           max_rel_residual=r64["max_rel_residual"], failed=r64["failed"],
+                           sweeps_hist=np.bincount(r64["sweeps"]).tolist()),
         "sweeps_hist": np.bincount(rmix["sweeps"]).tolist(),
-        "speedup_vs_plain_fp64": t64 / tmix,
+        "speedup_vs_plain_fp64": ((solved / tmix) / ((nsys - r64["failed"]) / t64)),
         "plain_fp64_per_system": per_sys,
         "speedup_vs_per_system_fp64": (per_sys["ms"] / (1e3 * tmix)) if "ms" in per_sys else None,
         "rel_residual_max": rmix["max_rel_residual"],
+        "rel_residual_median": float(np.median(rmix["rel_residual"])),
+        "fwd_err_vs_fp64": dict(max=float(ferr.max()), median=float(np.median(ferr)),
+                                note="per system ||x_mixed - x_fp64|| / ||x_fp64||, x_fp64 from the plain FP64 "
+                                     "batched solve of the same systems"),
         "failed_systems": rmix["failed"],
-        "refinement_sweeps_max": sweeps,
+        "failed_systems_plain_fp64": r64["failed"],
+        "refinement_sweeps_max": int(rmix["max_sweeps"]),
         "roofline": {
             "bound": "hbm",
             "kernel": f"{cls_names.get(dom, dom)}, FP32 factors, one launch per level",
@@ -352,7 +413,9 @@ def bench_batched(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_over_algorithmic": (traffic / (by_d / nl)) if traffic else None,
+            "traffic_source": "profiles/batched_uneven_4096.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-scaled)",
             "bytes_per_launch": by_d / nl,
             "avg_launch_ms": ms_d / nl,
             "launches": nl,
@@ -360,6 +423,7 @@ def bench_batched(args):
             "flop_frac_fp32_peak": fl_d / (ms_d * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
             "factorize_ms_by_class": {cls_names.get(k, str(k)): v[1] for k, v in sorted(per_cls.items())},
             "whole_solve_algorithmic_gbs": nbytes / tmix / 1e9,
+            "whole_solve_bytes_note": "factors once + per system (1 + its sweeps) x (solve + FP64 residual) bytes",
         },
         "device": device_info(),
     }
@@ -369,7 +433,181 @@ def bench_batched(args):
             result["cpu_baseline"] = run_cpu_baseline_batch(mix, dims, min(nsys, args.cpu_sample_sys))
         except Exception as e:  # the baseline is reported, never required
             result["cpu_baseline"] = {"error": str(e)}
+    del mix, X, X64, B
+    if not sel and not args.no_subs:
+        subs = {}
+        for name, fn in (("single_fp64_N100000_n32", sub_single_fp64),
+                         ("mixed_N20000_n64_kappa1e6_4rhs", sub_mixed_n64),
+                         ("selector_8192_systems", sub_selector)):
+            try:
+                t0 = time.perf_counter()
+                subs[name] = fn(args)
+                subs[name]["wall_s"] = time.perf_counter() - t0
+            except Exception as e:   # reported, never fatal for the headline line
+                subs[name] = {"error": str(e)}
+            print(f"[bench] {name}: {json.dumps(subs[name])[:300]}", file=sys.stderr, flush=True)
+        result["other_workloads"] = subs
     print(json.dumps(result))
+
+
+def _single_setup(N, n, qr, nr, shift, seed, kappa=0.0):
+    from alglin_amd import systems
+    from alglin_amd.bcr import DeviceArray, packed_blocks
+    if kappa > 0:
+        s = systems.conditioned_system(N, n, qr, qr, nr, nr, kmin=1e2, kmax=kappa, seed=seed)
+    else:
+        s = systems.gaussian_system(N, n, qr, qr, nr, nr, shift=shift, seed=seed)
+    pk = packed_blocks(s)
+    dev = {k: DeviceArray.from_host(v) for k, v in pk.items()}
+    return s, dev
+
+
+def _solver(s, dev, mode, pair_kernel=0):
+    from alglin_amd.bcr import BorderedCR
+    g = BorderedCR()
+    g.allocate(s["N"], s["n"], s["qr"], s["qx"], s["nr"], s["nx"])
+    g.select("LU", "LU", "NONE")
+    if mode == "mixed":
+        g.select_precision("MIXED", 1e-12, 30)
+    g.set_pair_kernel(pair_kernel)
+    g.bind_device(dev)
+    return g
+
+
+def _timed_solve(g, Xd, Bd, nrhs, L, steps):
+    from alglin_amd.bcr import device_synchronize
+    def step():
+        Xd.copy_from(Bd, stream=g.stream())
+        if not g.factorize_device(sync=False):
+            raise RuntimeError(g.last_error())
+        g.solve_device(Xd, nrhs, L, sync=True)
+    step()
+    device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    device_synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def sub_single_fp64(args):
+    """Plain FP64 on one large system: N = 100000, n = 32, qr = qx = 8,
+    nr = nx = 4, shift 3; 1 right-hand side (b = A 1) and 16 (randn)."""
+    from alglin_amd import systems
+    from alglin_amd.bcr import DeviceArray
+    N, n, q, r = 100000, 32, 8, 4
+    s, dev = _single_setup(N, n, q, r, 3.0, args.seed)
+    nrows, ncols = systems.dims(s)
+    L = max(nrows, ncols)
+    g = _solver(s, dev, "fp64")
+    b = np.zeros(L)
+    b[:nrows] = systems.mv(s, np.ones(ncols))
+    Bd, Xd = DeviceArray.from_host(b), DeviceArray(L)
+    g.timing(reset=True)
+    g.set_timing(True)
+    t1 = _timed_solve(g, Xd, Bd, 1, L, 5)
+    g.set_timing(False)
+    tm = g.timing(reset=True)
+    x = Xd.to_host()[:ncols]
+    r_ = DeviceArray(nrows)
+    g.residual_device(Xd, Bd, r_)
+    res = float(np.linalg.norm(r_.to_host()) / np.linalg.norm(b[:nrows]))
+    B16 = np.zeros((L, 16))
+    B16[:nrows] = np.random.default_rng(args.seed + 7).standard_normal((nrows, 16))
+    B16d = DeviceArray.from_host(np.asfortranarray(B16).reshape(-1, order="F"))
+    X16d = DeviceArray(L * 16)
+    t16 = _timed_solve(g, X16d, B16d, 16, L, 3)
+    launches = tm["pair_launches"] / 6
+    byts = (N - 1) * pair_bytes(n, r, r)
+    avg = tm["pair_ms"] / max(1.0, tm["pair_launches"])
+    return dict(ms_factor_solve_1rhs=1e3 * t1, solves_per_s=1.0 / t1, ms_factor_solve_16rhs=1e3 * t16,
+                rel_residual=res, max_fwd_err_vs_ones=float(np.abs(x - 1).max()),
+                pair_kernel_gbs=byts / launches / (avg * 1e-3) / 1e9,
+                pair_kernel_frac_hbm=byts / launches / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                pair_ms_avg_launch=avg)
+
+
+def sub_mixed_n64(args):
+    """Mixed precision on N = 20000, n = 64, qr = qx = 16, nr = nx = 8 with
+    kappa(D_k) log-uniform in [1e2, 1e6]: 4 random right-hand sides and
+    b = A 1, against plain FP64 on the same system (residuals and forward
+    error per column)."""
+    from alglin_amd import systems
+    from alglin_amd.bcr import DeviceArray
+    N, n, q, r = 20000, 64, 16, 8
+    s, dev = _single_setup(N, n, q, r, 0.0, args.seed, kappa=1e6)
+    nrows, ncols = systems.dims(s)
+    L = max(nrows, ncols)
+    Bh = np.zeros((L, 5))
+    Bh[:nrows, :4] = np.random.default_rng(args.seed + 7).standard_normal((nrows, 4))
+    Bh[:nrows, 4] = systems.mv(s, np.ones(ncols))
+    Bd = DeviceArray.from_host(np.asfortranarray(Bh).reshape(-1, order="F"))
+    B1 = DeviceArray.from_host(Bh[:, 4].copy())
+    out = {}
+    X = {}
+    for mode in ("fp64", "mixed"):
+        g = _solver(s, dev, mode)
+        Xd = DeviceArray(L * 5)
+        t5 = _timed_solve(g, Xd, Bd, 5, L, 3)
+        X[mode] = Xd.to_host().reshape(L, 5, order="F")[:ncols]
+        X1 = DeviceArray(L)
+        t1 = _timed_solve(g, X1, B1, 1, L, 3)
+        out[mode] = dict(ms_4rhs_plus_A1=1e3 * t5, ms_1rhs_A1=1e3 * t1)
+        if mode == "mixed":
+            out[mode]["refine_report"] = g.refine_report()
+        g.close()
+    res = {m: [float(np.linalg.norm(systems.mv(s, X[m][:, c]) - Bh[:nrows, c]) / np.linalg.norm(Bh[:nrows, c]))
+               for c in range(5)] for m in X}
+    out["rel_residual_per_column"] = res
+    out["fwd_err_mixed_vs_fp64_per_column"] = [float(np.linalg.norm(X["mixed"][:, c] - X["fp64"][:, c]) /
+                                                     np.linalg.norm(X["fp64"][:, c])) for c in range(5)]
+    out["fwd_err_vs_ones_A1_column"] = {m: float(np.abs(X[m][:, 4] - 1).max()) for m in X}
+    out["note"] = ("columns 0-3 randn, column 4 b = A 1; plain FP64 cannot reach 1e-12 on the random columns "
+                   "(kappa up to 1e6); the mixed solver stops them at the FP64 floor (status 5)")
+    return out
+
+
+def sub_selector(args):
+    """8192 systems, n in {8,16,32}, N in [64,2048], half with a nearly
+    singular pair column: mixed + learned selector + FP64 fallback against
+    the same without the selector and against plain FP64."""
+    from alglin_amd import selector as S
+    from alglin_amd.bcr import Batch, DeviceArray
+    nsys, tol = 8192, 1e-12
+    dims = S.selector_dims(nsys, args.seed)
+    sing = S.singular_pairs(dims, args.seed)
+    ref = Batch(mixed=False)
+    ref.plan(dims)
+    ref.generate(seed=args.seed, shift=4.0, sing_k=sing, eps=1e-10)
+    mix = Batch(mixed=True)
+    mix.plan(dims, operator=False)
+    mix.share_operator(ref)
+    L = ref.size("x")
+    B = DeviceArray.from_host(np.random.default_rng(args.seed + 5).standard_normal(L))
+    X = DeviceArray(L)
+    mlp = S.load()
+
+    def timed(fn, steps=3):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t0) / steps
+
+    t64 = timed(lambda: ref.solve(B, X, tol, 30, graph=True))
+    f64 = ref.report()["failed"]
+    del ref
+    t_none = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=None, fallback=True))
+    f_none = mix.report()["failed"]
+    t_sel = timed(lambda: mix.solve_selected(B, X, tol, 30, selector=mlp, fallback=True))
+    r_sel = mix.report()
+    ch = mix.selection()
+    return dict(plain_fp64=dict(ms=1e3 * t64, solves_per_s=(nsys - f64) / t64, failed=f64),
+                mixed_without_selector=dict(ms=1e3 * t_none, solves_per_s=(nsys - f_none) / t_none, failed=f_none),
+                mixed_learned=dict(ms=1e3 * t_sel, solves_per_s=(nsys - r_sel["failed"]) / t_sel,
+                                   failed=r_sel["failed"], flagged=int(ch["chose_qrp"].sum()),
+                                   fp64_path=int(ch["fell_back"].sum())),
+                speedup_learned_vs_plain_fp64=((nsys - r_sel["failed"]) / t_sel) / ((nsys - f64) / t64))
 
 
 def main():
@@ -393,7 +631,9 @@ def main():
                     help="single: one large system; batched: 4096 uneven systems; selector: 8192 systems with "
                          "the learned LU/QRP choice")
     ap.add_argument("--nsys", type=int, default=0)
-    ap.add_argument("--cpu-sample-sys", type=int, default=48)
+    ap.add_argument("--cpu-sample-sys", type=int, default=512)
+    ap.add_argument("--no-subs", action="store_true",
+                    help="batched workload: skip the single-system, n = 64 mixed and selector sub-results")
     ap.add_argument("--per-system-sample", type=int, default=256,
                     help="systems timed one at a time through the plain FP64 BorderedCR (batched workloads)")
     ap.add_argument("--pair-kernel", type=int, default=0,

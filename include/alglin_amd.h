@@ -92,6 +92,26 @@ int alglin_bcr_sparse_load(alglin_bcr_t h, void const* V, int32_t const* I, int3
 int alglin_bcr_check_matrix(alglin_bcr_t h);                                 /* cc:2538-2659 */
 char const* alglin_bcr_last_error(alglin_bcr_t h);                           /* hxx:445 */
 char const* alglin_bcr_info(alglin_bcr_t h);                                 /* cc:208-235 */
+/* debug mode (hxx:1172): factorize scans the blocks for NaN / Inf first
+   (check_matrix, cc:2538-2659) and fails on the first one */
+int alglin_bcr_set_debug(alglin_bcr_t h, int on);
+/* elimination order (hxx:407-423, cc:139-158): npb block ranges reduced on
+   their own, then the chain of their end rows; 1 = pure cyclic reduction.
+   Applies from the next factorization. */
+int alglin_bcr_set_num_parallel_block(alglin_bcr_t h, int32_t npb);
+/* the partition count in use after the reference's shrink rule (2P <= N) */
+int32_t alglin_bcr_used_parallel_block(alglin_bcr_t h);
+/* copy of another handle's loaded blocks and partition (cc:181-198) */
+int alglin_bcr_dup(alglin_bcr_t h, alglin_bcr_t src);
+/* transposed solve: not defined in the reference either, fails with its
+   message (cc:2089-2101) */
+int alglin_bcr_t_solve(alglin_bcr_t h, void* x);
+/* print_matlab_script (cc:2493-2534) into buf (len bytes, NUL-terminated,
+   truncated); returns the length the whole script needs, -1 on error */
+int64_t alglin_bcr_matlab_script(alglin_bcr_t h, char* buf, int64_t len);
+/* host copies of the loaded blocks, packed as in alglin_bcr_load_packed;
+   any pointer may be null */
+int alglin_bcr_get_packed(alglin_bcr_t h, void* D, void* E, void* B, void* C, void* H, void* Cq, void* F);
 
 /* ---- device-resident variants (HIP device pointers) ------------------------ */
 /* use caller-owned device arrays (packed layout as above) as the operator */
@@ -99,9 +119,13 @@ int alglin_bcr_bind_device(alglin_bcr_t h, void const* D, void const* E, void co
                            void const* H, void const* Cq, void const* F);
 /* enqueue the factorization of the bound operator; sync=1 waits and returns status */
 int alglin_bcr_factorize_device(alglin_bcr_t h, int sync);
-/* in-place solve of a device block X (ld x nrhs) */
+/* in-place solve of a device block X (ld x nrhs, ld >= max(nrows, ncols)
+   when nrhs > 1; a single column needs max(nrows, ncols) entries); fails
+   when the object is not factorized */
 int alglin_bcr_solve_device(alglin_bcr_t h, void* X, int32_t nrhs, int64_t ld, int sync);
-/* FP64 residual r = b - A x on the bound operator (double device vectors) */
+/* FP64 residual r = b - A x on the operator the handle factors (double
+   device vectors; the mixed engine's copy after a MIXED factorize of host
+   blocks) */
 int alglin_bcr_residual_device(alglin_bcr_t h, double const* x, double const* b, double* r, int sync);
 int alglin_bcr_sync(alglin_bcr_t h);
 /* the HIP stream the handle launches on (hipStream_t) */
@@ -128,14 +152,20 @@ int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant);
    only): FP32 cyclic-reduction factors + FP64 iterative refinement on the GPU
    until ||b - A x||_2 / ||b||_2 <= tol or max_iter correction sweeps */
 int alglin_bcr_set_precision(alglin_bcr_t h, char const* mode, double tol, int32_t max_iter);
-/* MIXED: when refinement stalls, solve again by FP64 cyclic reduction with
-   column-pivoted QR on the original operator (on by default); off = the solve
-   fails with "iterative refinement did not reach the requested residual" */
+/* MIXED: the columns whose refinement stalls (or hits max_iter) are solved
+   again by FP64 cyclic reduction with column-pivoted QR on the original
+   operator, then refined in FP64 (on by default); off = the solve fails with
+   "iterative refinement did not reach the requested residual" */
 int alglin_bcr_set_fallback(alglin_bcr_t h, int on);
-/* last refined solve: out = {sweeps, rel. residual, residual after the first
-   solve, status (0 converged, 1 stalled, 2 max_iter, 3 factorization failed,
-   4 stalled and solved by the FP64 QRP fallback)} (worst over columns) */
-int alglin_bcr_refine_report(alglin_bcr_t h, double out[4]);
+/* last refined solve, worst over the columns: out = {sweeps, rel. residual,
+   residual after the first solve, status, componentwise backward error
+   max_i |b - A x|_i / (|A||x| + |b|)_i where evaluated (-1: never)}.
+   status: 0 converged to tol; 1 stalled (no progress, backward error above
+   64 u); 2 max_iter; 3 the FP64 fallback factorization failed; 4 solved to
+   tol by the FP64 QRP fallback; 5 stopped at the FP64 residual floor
+   (backward error <= 64 u: no FP64 method gets the residual lower); 6 the
+   FP64 QRP fallback still missed tol.  0, 4 and 5 count as solved. */
+int alglin_bcr_refine_report(alglin_bcr_t h, double out[5]);
 
 /* ---- per-phase GPU timing (HIP events on the handle's stream) --------------- */
 /* on=1 records an event pair around every launch group from now on */
