@@ -72,6 +72,7 @@ def lib():
             "alglin_bcr_set_fallback": (ci, [vp, ctypes.c_int]),
             "alglin_bcr_set_pair_kernel": (ci, [vp, ctypes.c_int]),
             "alglin_bcr_timing": (ci, [vp, vp, ci]),
+            "alglin_bcr_factor_solve_noalloc": (ci, [vp] + [vp] * 8 + [i32, i64, vp, ci, ctypes.c_char_p, i32]),
             "alglin_dev_malloc": (vp, [i64]),
             "alglin_dev_free": (ci, [vp]),
             "alglin_memcpy_htod": (ci, [vp, vp, i64]),

@@ -177,6 +177,28 @@ int alglin_bcr_set_timing(alglin_bcr_t h, int on);
    out[7] launches of sweep kernels; reset=1 clears after reading */
 int alglin_bcr_timing(alglin_bcr_t h, double out[8], int reset);
 
+/* ---- allocation-free factor + solve -------------------------------------------
+   No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  The same FP64 cyclic reduction (LU eliminations, LU
+   last block, pure cyclic-reduction order) and the same kernels as
+   alglin_bcr_bind_device + alglin_bcr_factorize_device +
+   alglin_bcr_solve_device, with every working array a slice of one fixed
+   64 MiB device array of the library (one per GPU, held until the process
+   ends): no hipMalloc, no hipFree.  dims = {N, n, qr, qx, nr, nx}
+   (qr + nr == qx + nx, n <= 64, nr <= 16); D..F: the operator in the packed
+   layout of alglin_bcr_load_packed (device pointers, not modified); X: ld x
+   nrhs device block, right-hand sides in, solutions out (ld >= max(nrows,
+   ncols) when nrhs > 1); stream: a hipStream_t (0 = the null stream).
+   sync = 1 waits and reports the factorization status; sync = 0 only enqueues.
+   Returns 0 ok, 1 zero pivot, 2 singular last block, 3 the system's working
+   set exceeds the 64 MiB array, -1 bad arguments; err (errlen bytes, may be
+   null) receives the message.  Calls on one GPU share the array: calls that
+   overlap on one GPU race. */
+int alglin_bcr_factor_solve_noalloc(int32_t const dims[6], double const* D, double const* E, double const* B,
+                                    double const* C, double const* H, double const* Cq, double const* F, double* X,
+                                    int32_t nrhs, int64_t ld, void* stream, int sync, char* err, int32_t errlen);
+
 /* ---- device memory (no other runtime needed) -------------------------------- */
 void* alglin_dev_malloc(int64_t bytes);
 int alglin_dev_free(void* p);
