@@ -14,6 +14,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libalglin_amd.so")
+# per-class kernel choices of the autotuner (alglin_amd/autotune.py)
+TUNING_PATH = os.path.join(_HERE, "tuning.json")
 _lib = None
 
 
@@ -81,6 +83,7 @@ def lib():
             "alglin_memcpy_dtod_async": (ci, [vp, vp, i64, vp]),
             "alglin_batch_time_factorize": (ci, [vp, vp, i32]),
             "alglin_batch_set_pair_kernel": (ci, [vp, ctypes.c_int]),
+            "alglin_batch_set_class_kernel": (ci, [vp, ci, ci, ci, ci]),
             "alglin_batch_create": (vp, [ctypes.c_int]),
             "alglin_batch_destroy": (None, [vp]),
             "alglin_batch_plan": (ci, [vp, i32, vp]),
@@ -318,7 +321,7 @@ class BorderedCR:
         self._check(lib().alglin_bcr_set_precision(self._h, mode.encode(), tol, max_iter), "select_precision")
 
     def set_pair_kernel(self, variant: int):
-        """LU pair kernel: 0 default, 2 two-role kernel (16 < n <= 32), 3 register tiles for every n,
+        """LU pair kernel: 0 default, 3 register tiles for every n,
         4 column-split waves for every n <= 64, 5 the matrix-core pair (FP32 factors,
         n <= 64, nr <= 8, nx <= 16).  The default takes the matrix-core pair for FP32
         factors with 16 < n <= 64 where it applies."""
@@ -493,12 +496,16 @@ class Batch:
     def last_error(self) -> str:
         return lib().alglin_batch_last_error(self._h).decode()
 
-    def plan(self, dims, operator: bool = True):
+    def plan(self, dims, operator: bool = True, tuning: bool = True):
         """Lays the systems out; with operator=False no operator storage is
-        allocated (share_operator must follow before any solve)."""
+        allocated (share_operator must follow before any solve).  tuning:
+        apply the autotuner's per-class kernel choices (alglin_amd/tuning.json)
+        when the file exists."""
         d = np.ascontiguousarray(np.asarray(dims, np.int32).reshape(-1, 7))
         self.dims = d
         self._check(lib().alglin_batch_plan(self._h, len(d), _p(d)), "plan")
+        if tuning:
+            self.load_tuning()
         # operator storage (device, packed)
         if self.ops is not None and not getattr(self, "_shared", False):
             for a in self.ops.values():
@@ -617,6 +624,25 @@ class Batch:
         rank-1 kernels (two-wave split, column-split), 5 the matrix-core pair
         (FP32 factors, n <= 48, nr <= 8)."""
         self._check(lib().alglin_batch_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
+
+    def set_class_kernel(self, cls: int, kernel: int, waves: int = 0, fp32_factors=None):
+        """Kernel of LU pair class cls (3: 16 < n <= 32, 4: 32 < n <= 64):
+        -1 default, 0 matrix-core pair, 1 rank-1 (column-split with `waves`
+        waves per pair for class 4), 2 register-tiled (class 4)."""
+        fp32 = self.mixed if fp32_factors is None else bool(fp32_factors)
+        self._check(lib().alglin_batch_set_class_kernel(self._h, int(fp32), cls, kernel, waves), "set_class_kernel")
+
+    def load_tuning(self, path=None):
+        """Apply the per-class kernel choices the autotuner stored (JSON)."""
+        import json
+        path = path or TUNING_PATH
+        if not os.path.exists(path):
+            return False
+        t = json.load(open(path))
+        for key, prec in (("fp32_factors", True), ("fp64_factors", False)):
+            for cls, c in t.get(key, {}).items():
+                self.set_class_kernel(int(cls), int(c["kernel"]), int(c.get("waves", 0)), fp32_factors=prec)
+        return True
 
     def time_factorize(self) -> np.ndarray:
         """One eager factorization with every pair-elimination launch timed

@@ -427,6 +427,7 @@ bool BorderedCR<T>::factorize() {
   try {
     if (debug_) check_matrix();
     if (d_N_ < 1) throw std::runtime_error("BorderedCR::factorize, not allocated");
+    check_supported();
     Gpu& g = gpu();
     int32_t st = 0;
     device_bound_ = false;   // the host blocks become the operator
@@ -549,8 +550,22 @@ void BorderedCR<T>::bind_device(T const* D, T const* E, T const* B, T const* C, 
 }
 
 template <typename T>
+void BorderedCR<T>::check_supported() const {
+  // the per-pair LDS tables of the sweep and generic kernels hold 64 pivots
+  // and 16 border rows
+  if (d_n_ > 64 || d_nr_ > 16)
+    throw std::runtime_error("BorderedCR::factorize, block size n <= 64 and nr <= 16 are supported (n = " +
+                             std::to_string(d_n_) + ", nr = " + std::to_string(d_nr_) + ")");
+}
+
+template <typename T>
 int32_t BorderedCR<T>::factorize_device(bool sync) {
   if (!device_bound_) fail("BorderedCR::factorize_device: no device operator bound");
+  try {
+    check_supported();
+  } catch (std::exception const& e) {
+    fail(e.what());
+  }
   Gpu& g = gpu();
   int32_t st = 0;
   if (mixed_) {

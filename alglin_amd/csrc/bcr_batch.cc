@@ -2,6 +2,7 @@
 #include "bcr_batch.hh"
 
 #include "bcr_mlp.hh"
+#include "bcr_refine.hh"
 
 #include <algorithm>
 #include <chrono>
@@ -58,7 +59,6 @@ int factor_class(SysDims const& d) {
   if (d.n <= 32 && d.nx <= 8 && d.nr <= 8) return 3;
   return 4;
 }
-constexpr int kFacClasses = 7;
 constexpr int kFastNmax[3] = {4, 8, 16};
 
 // sweep kernel class: 0..3 wave kernels for LU with n <= 4, 8, 16, 32;
@@ -252,6 +252,14 @@ void BatchEngine<TA, TW>::widen() {
   stash_.mv.clear();
   stash_.norm.clear();
   stash_.add.clear();
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::set_class_kernel(int cls, int kernel, int waves) {
+  if (cls < 0 || cls >= kFacClasses) throw HipError("set_class_kernel: bad class");
+  if (kernel < -1 || kernel > kKernTiled) throw HipError("set_class_kernel: bad kernel");
+  cls_kernel_[cls] = kernel;
+  cls_waves_[cls] = waves;
 }
 
 template <typename TA, typename TW>
@@ -456,12 +464,16 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
         using A = std::remove_cv_t<std::remove_pointer_t<decltype(args)>>;
         BatchTable<A> t{args, l.entry, l.first};
         bool ok = true;
+        // the class's kernel: a tuned choice (set_class_kernel) or the
+        // default of pair_variant_
+        const int ck = cls_kernel_[l.cls];
+        const int v = ck == kKernMfma ? 5 : ck >= 0 ? 4 : pair_variant_;
         if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
-        else if ((l.cls == 3 || l.cls == 4) && try_mfma(pair_variant_, t, l, s, &e))
+        else if ((l.cls == 3 || l.cls == 4) && try_mfma(v, t, l, s, &e))
           ok = true;
         else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, s, &e);
-        else if (l.cls == 4 && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
-          ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
+        else if (l.cls == 4 && ck != kKernTiled && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
+          ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e, cls_waves_[l.cls]);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
           ok = launch_pair_lu_tiled_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
         else {
@@ -681,15 +693,25 @@ typename BatchEngine<TA, TW>::MvTables& BatchEngine<TA, TW>::mv_tables(double co
 }
 
 template <typename TA, typename TW>
-void BatchEngine<TA, TW>::residual_async(double const* X, double const* B, double* R) {
+void BatchEngine<TA, TW>::residual_async(double const* X, double const* B, double* R, bool absval) {
   MvTables& T = mv_tables(X, B, R);
   const int ns = int(active_.size());
   if (ns == 0) return;
   BatchTable<MvArgs<TA>> t{static_cast<const MvArgs<TA>*>(T.rows.args), T.rows.entry, T.rows.first};
-  hip_check(launch_mv_rows_batched<TA>(t, T.rows.nblocks, st_), "batch mv rows");
+  hip_check(launch_mv_rows_batched<TA>(t, T.rows.nblocks, st_, absval), "batch mv rows");
   hip_check(launch_convert<double, double>(nullptr, Csum_.p, int64_t(Csum_.count), st_), "zero Csum");
   hip_check(launch_reduce_batched<double, double>(T.csum, ns, st_), "reduce Csum");
-  hip_check(launch_mv_tail_batched<TA>(T.tail, T.csum_ptr, ns, st_), "batch mv tail");
+  hip_check(launch_mv_tail_batched<TA>(T.tail, T.csum_ptr, ns, st_, absval), "batch mv tail");
+}
+
+template <typename TA, typename TW>
+void BatchEngine<TA, TW>::backward_errors_async(double const* X, double const* B, double const* R, double* den,
+                                                double* out) {
+  residual_async(X, B, den, true);   // den = |A| |X| + |B|
+  norms_async(den, R, nullptr);      // the per-system (den, R) table
+  auto it = norm_cache_.find(std::vector<const void*>{den, R});
+  if (!active_.empty())
+    hip_check(launch_berr_batched(it->second.args, int(active_.size()), out, st_), "batch backward errors");
 }
 
 template <typename TA, typename TW>
@@ -882,6 +904,20 @@ void BatchSolver::set_pair_variant(int v) {
   if (e64_) e64_->set_pair_variant(v);
 }
 
+void BatchSolver::set_class_kernel(bool fp32_factors, int cls, int kernel, int waves) {
+  if (cls < 0 || cls >= kFacClasses) throw HipError("set_class_kernel: bad class");
+  drop_graphs();
+  auto& t = fp32_factors ? tune32_ : tune64_;
+  t[cls][0] = kernel;
+  t[cls][1] = waves;
+  if (fp32_factors && e32_) e32_->set_class_kernel(cls, kernel, waves);
+  if (!fp32_factors && e64_) e64_->set_class_kernel(cls, kernel, waves);
+}
+
+void BatchSolver::apply_tuning(BatchEngine<double, double>& e) {
+  for (int c = 0; c < kFacClasses; ++c) e.set_class_kernel(c, tune64_[c][0], tune64_[c][1]);
+}
+
 void BatchSolver::set_methods(std::vector<int32_t> const& m) {
   drop_graphs();
   for (size_t s = 0; s < m.size() && s < dims_.size(); ++s) {
@@ -919,7 +955,7 @@ hipGraphExec_t capture(hipStream_t st, F&& body) {
 // of open systems shrinks; one int (the open count) comes back per sweep.
 template <typename E>
 void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done,
-                         BatchReport& rep, int giveup) {
+                         BatchReport& rep, int giveup, bool have_residual) {
   const int ns = eng.nsys();
   hipStream_t st = eng.stream();
   // the captured lists belong to one engine: the FP64 fallback engine runs eager
@@ -936,7 +972,9 @@ void BatchSolver::refine(E& eng, double const* B, double* X, double tol, int max
   int32_t open = 0;
   bool narrowed = false;
   for (int it = 0;; ++it) {
-    if (graph && !narrowed) {
+    if (it == 0 && have_residual) {
+      // the caller's residual of the first solve stands
+    } else if (graph && !narrowed) {
       hip_check(hipGraphLaunch(g_.res, st), "graph residual");
     } else {
       eng.residual_async(X, B, R_.p);
@@ -1000,6 +1038,43 @@ void BatchSolver::ensure_graphs(E& eng, double const* B, double* X) {
   g_eng_ = &eng;
 }
 
+// Systems that ended above tol without a factorization failure: their
+// componentwise backward error max_i |r_i| / (|A| |x| + |b|)_i, from the
+// residual the refinement left in R_ (a system leaves the sweeps with its
+// last residual in place).  omega <= 64 u marks the FP64 floor: x solves a
+// system within a few roundings of (A, b), no FP64 path gets the residual
+// lower, and the FP64 fallback is not tried.  Returns the floor count.
+template <typename E>
+int BatchSolver::mark_floor(E& eng, double const* B, double* X, double tol) {
+  const int ns = eng.nsys();
+  floor_.assign(ns, 0);
+  std::vector<char> cand(ns, 0);
+  int k = 0;
+  for (int s = 0; s < ns; ++s)
+    if (status_[s] == 0 && rel_[s] == rel_[s] && !(rel_[s] <= tol)) {
+      cand[s] = 1;
+      ++k;
+    }
+  if (k == 0) return 0;
+  hipStream_t st = eng.stream();
+  eng.narrow(cand);   // tables of the candidates only; widen() restores the others
+  hip_check(launch_convert<int32_t, int32_t>(nullptr, eng.skip_flags(), ns, st), "clear skip flags");
+  den_.resize(size_t(std::max<int64_t>(eng.total(kbX), 1)));
+  berr_.resize(size_t(ns));
+  eng.backward_errors_async(X, B, R_.p, den_.p, berr_.p);
+  std::vector<double> om(ns, 0.0);
+  hip_check(hipMemcpyAsync(om.data(), berr_.p, sizeof(double) * size_t(ns), hipMemcpyDeviceToHost, st), "omega");
+  hip_check(hipStreamSynchronize(st), "omega sync");
+  eng.widen();
+  int f = 0;
+  for (int s = 0; s < ns; ++s)
+    if (cand[s] && om[s] <= MixedEngine::kFloorBerr) {
+      floor_[s] = 1;
+      ++f;
+    }
+  return f;
+}
+
 template <typename E>
 BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph) {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1026,9 +1101,11 @@ BatchReport BatchSolver::run(E& eng, double const* B, double* X, double tol, int
   std::vector<char> done(ns, 0);
   BatchReport rep;
   refine(eng, B, X, tol, max_iter, done, rep);
+  mark_floor(eng, B, X, tol);
   for (int s = 0; s < ns; ++s) {
     rep.max_rel_residual = std::max(rep.max_rel_residual, rel_[s]);
     if (status_[s] == 0 && !(rel_[s] <= tol)) status_[s] |= 4;
+    if (floor_[s]) status_[s] |= 8;
     if (status_[s] != 0) ++rep.failed;
   }
   rep.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1121,7 +1198,11 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
   // behind it, a system that would need more than six further sweeps at its
   // contraction rate is handed over at once instead of sweeping to max_iter
   std::vector<char> done(direct.begin(), direct.end());
-  refine(eng, B, X, tol, max_iter, done, rep, fallback ? 6 : 0);
+  refine(eng, B, X, tol, max_iter, done, rep, fallback ? 6 : 0, /*have_residual=*/true);
+  // systems at the FP64 floor stay where they are (no FP64 path does better;
+  // the systems sent straight to the FP64 path never ran the sweeps and
+  // carry no residual, so they are not candidates)
+  mark_floor(eng, B, X, tol);
   // 4. the FP64 fallback for the systems that did not get there: first the
   // plain FP64 path (LU elimination, FP64 refinement), then FP64 QRP for what
   // FP64 LU cannot solve either (QRP's factors lose digits on nearly singular
@@ -1130,7 +1211,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
   auto failed = [&](std::vector<char> const& among, std::vector<char>& out) {
     int k = 0;
     for (int s = 0; s < ns; ++s) {
-      out[s] = among[s] && (status_[s] != 0 || !(rel_[s] <= tol));
+      out[s] = among[s] && !floor_[s] && (status_[s] != 0 || !(rel_[s] <= tol));
       k += out[s];
     }
     return k;
@@ -1147,6 +1228,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       e64_ = new BatchEngine<double, double>();
       e64_->plan(dims_);
       e64_->bind_device(oD_, oE_, oB_, oC_, oH_, oCq_, oF_);
+      apply_tuning(*e64_);
     }
     auto& f = *e64_;
     hip_check(hipStreamSynchronize(st), "sync before fallback");
@@ -1172,11 +1254,11 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       if (stage == 1 && failed(bad, still) == 0) break;
       bad.swap(still);
     }
-    f.set_active(all);
   }
   for (int s = 0; s < ns; ++s) {
     rep.max_rel_residual = std::max(rep.max_rel_residual, rel_[s]);
     if (status_[s] == 0 && !(rel_[s] <= tol)) status_[s] |= 4;
+    if (floor_[s]) status_[s] |= 8;
     if (status_[s] != 0) ++rep.failed;
   }
   rep.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

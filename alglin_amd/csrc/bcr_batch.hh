@@ -29,6 +29,10 @@
 
 namespace bcr {
 
+// factorization kernel classes of a batch: 0..4 LU by block size (n <= 4, 8,
+// 16, 32, above), 5 QR, 6 QRP
+constexpr int kFacClasses = 7;
+
 struct SysDims {
   int32_t N, n, qr, qx, nr, nx;
   int32_t method;   // kLU, kQR, kQRP (elimination of the pairs)
@@ -59,6 +63,12 @@ class BatchEngine {
   // pair elimination method of system s (kLU, kQR, kQRP)
   void set_method(int s, int method);
   int method(int s) const { return dims_[size_t(s)].method; }
+  // kernel of an LU pair class (3: 16 < n <= 32, 4: 32 < n <= 64): -1 the
+  // default, kKernMfma the matrix-core pair (FP32 factors), kKernRank1 the
+  // rank-1 kernels (two-wave split / column-split with `waves` waves per
+  // pair, 0 = default), kKernTiled the register-tiled kernel (class 4)
+  static constexpr int kKernMfma = 0, kKernRank1 = 1, kKernTiled = 2;
+  void set_class_kernel(int cls, int kernel, int waves);
   size_t lds_limit() const { return lds_limit_; }
   // false when the pair panel of d would need LDS beyond lds_limit (the
   // shapes no register kernel of its class covers)
@@ -79,8 +89,12 @@ class BatchEngine {
   void status(int32_t* host_out);
   // in-place solve of the packed vector X (offsets kbX, 1 right-hand side)
   void solve_async(double* X);
-  // R = B - A X per system (FP64, pristine operator)
-  void residual_async(double const* X, double const* B, double* R);
+  // R = B - A X, or (absval) |A| |X| + |B|, per active system (FP64, the
+  // pristine operator)
+  void residual_async(double const* X, double const* B, double* R, bool absval = false);
+  // out[s] = max_i |R_i| / (|A| |X| + |B|)_i per active system (the
+  // componentwise backward error; den: x-layout scratch)
+  void backward_errors_async(double const* X, double const* B, double const* R, double* den, double* out);
   // out[2s] = ||B_s||, out[2s+1] = ||R_s|| (device array of 2*nsys)
   void norms_async(double const* B, double const* R, double* out);
 
@@ -170,6 +184,8 @@ class BatchEngine {
   std::vector<int64_t> offT_, offPerm_, offCpend_, offFpart_, offHm_, offPiv_, offXpart_, offCpart_, offCsum_;
   int maxN_ = 0, levels_ = 0, Ncmax_ = 0, Nrmax_ = 0;
   size_t lds_limit_ = 65536, factor_bytes_ = 0;
+  int cls_kernel_[kFacClasses] = {-1, -1, -1, -1, -1, -1, -1};
+  int cls_waves_[kFacClasses] = {0, 0, 0, 0, 0, 0, 0};
   hipStream_t st_ = nullptr;
   // A second stream for the n > 32 class: the classes touch disjoint
   // systems, so their level sequences run concurrently (forked from and
@@ -260,6 +276,9 @@ class BatchSolver {
   void set_methods(std::vector<int32_t> const& m);
   // kernel of the 32 < n <= 64 LU class (BatchEngine::set_pair_variant)
   void set_pair_variant(int v);
+  // tuned kernel of an LU pair class for the FP32-factor engine or the FP64
+  // one (BatchEngine::set_class_kernel); kept for an engine created later
+  void set_class_kernel(bool fp32_factors, int cls, int kernel, int waves);
   std::vector<int32_t> const& status() const { return status_; }
   std::vector<double> const& rel_residual() const { return rel_; }
   std::vector<int32_t> const& sweeps() const { return sweeps_; }
@@ -272,7 +291,13 @@ class BatchSolver {
  private:
   template <typename E> BatchReport run(E& eng, double const* B, double* X, double tol, int max_iter, bool use_graph);
   template <typename E> void ensure_graphs(E& eng, double const* B, double* X);
+  template <typename E> int mark_floor(E& eng, double const* B, double* X, double tol);
+  std::vector<char> floor_;          // per system: stopped at the FP64 residual floor
+  DevBuf<double> den_, berr_;        // |A||x| + |b| rows, per-system backward errors
   bool graphs_ = true;   // solve_selected replays captured launch lists
+  int tune32_[kFacClasses][2] = {{-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}};
+  int tune64_[kFacClasses][2] = {{-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}};
+  void apply_tuning(BatchEngine<double, double>& e);
   const void* g_eng_ = nullptr;   // the engine the captured lists launch on
   bool mixed_;
   BatchEngine<double, double>* e64_ = nullptr;
@@ -290,10 +315,12 @@ class BatchSolver {
   double const *oD_ = nullptr, *oE_ = nullptr, *oB_ = nullptr, *oC_ = nullptr, *oH_ = nullptr, *oCq_ = nullptr,
                *oF_ = nullptr;
   // refinement loop over the systems of `eng` not marked done (device
-  // convergence test; done systems are skipped by every launch)
+  // convergence test; done systems are skipped by every launch).
+  // have_residual: R_ and norms_ already hold b - A x of the current X, so
+  // the first sweep starts with the convergence test
   template <typename E>
   void refine(E& eng, double const* B, double* X, double tol, int max_iter, std::vector<char>& done, BatchReport& rep,
-              int giveup = 0);
+              int giveup = 0, bool have_residual = false);
   // captured graphs: factor+solve, residual+norms, correction
   struct Graphs {
     const double* B = nullptr;

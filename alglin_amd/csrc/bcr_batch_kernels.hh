@@ -94,6 +94,9 @@ hipError_t launch_prep_batched(const PrepArgs<TA, TW>* args, int nsys, hipStream
 template <typename TP, typename TD>
 hipError_t launch_reduce_batched(const ReduceArgs<TP, TD>* args, int nsys, hipStream_t st);
 hipError_t launch_norms_batched(const NormArgs* args, int nsys, double* out, hipStream_t st);
+// out[sys] = max_i |r_i| / b_i over the system's rows (b = |A||x| + |b|:
+// the componentwise backward error; 0 / 0 counts as 0)
+hipError_t launch_berr_batched(const NormArgs* args, int nsys, double* out, hipStream_t st);
 hipError_t launch_gen_batched(const GenArgs* args, int nsys, uint64_t seed, hipStream_t st);
 
 // pair elimination (factor), one launch per reduction level and kernel class
@@ -113,15 +116,13 @@ bool launch_pair_lu_split_batched(BatchTable<FactorArgs<TS, TW>> const& t, int n
 template <typename TS, typename TW>
 bool launch_pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                   hipStream_t st, hipError_t* err);
+// waves per pair: 0 default, else 4 / 8 / 16 (n <= 48) or 8 / 16 (n <= 64)
 template <typename TS, typename TW>
 bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
-                                 hipStream_t st, hipError_t* err);
+                                 hipStream_t st, hipError_t* err, int waves = 0);
 template <typename TS>
 bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err);
-template <typename TS, typename TW>
-bool launch_pair_lu_tw_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
-                               hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 hipError_t launch_pair_generic_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, size_t bytes,
                                        size_t lds_limit, hipStream_t st);
@@ -149,8 +150,9 @@ hipError_t launch_last_solve_batched(const LastSolveArgs<TX>* args, int nsys, in
 
 // residual SpMV
 template <typename TA>
-hipError_t launch_mv_rows_batched(BatchTable<MvArgs<TA>> const& t, int nblocks, hipStream_t st);
+hipError_t launch_mv_rows_batched(BatchTable<MvArgs<TA>> const& t, int nblocks, hipStream_t st, bool absval = false);
 template <typename TA>
-hipError_t launch_mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum, int nsys, hipStream_t st);
+hipError_t launch_mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum, int nsys, hipStream_t st,
+                                  bool absval = false);
 
 }  // namespace bcr

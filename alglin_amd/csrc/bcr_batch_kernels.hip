@@ -132,22 +132,35 @@ __global__ void __launch_bounds__(256) gen_batched(const GenArgs* args, int kind
   }
 }
 
-// one workgroup per system
+template <bool MAX>
+__device__ __forceinline__ double wave_minmax(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = MAX ? fmax(v, o) : fmin(v, o);
+  }
+  return v;
+}
+
+// one workgroup per system, its four waves taking the pair panels in turn:
+// per panel the lanes read the pivots (the diagonal of U) and, column by
+// column, the upper triangle (coalesced within a column), so a panel costs
+// about n loads per lane instead of n(n+1)/2 loads of one thread
 template <typename TW>
 __global__ void __launch_bounds__(256) features_batched(const FeatArgs<TW>* args, float* out) {
-  __shared__ double r_min[256], r_max[256], r_sum[256], r_rat[256], r_cnt[256], r_up[256];
+  __shared__ double r_min[4], r_max[4], r_sum[4], r_rat[4], r_cnt[4], r_up[4];
   const FeatArgs<TW>& a = args[blockIdx.x];
-  const int n = a.n, n2 = 2 * n, t = threadIdx.x;
+  const int n = a.n, n2 = 2 * n, t = threadIdx.x, lane = t & 63, w = t >> 6;
+  // wave-uniform accumulators over this wave's panels; umax per lane
   double dmin = INFINITY, dmax = 0, lsum = 0, rmin = INFINITY, cnt = 0, umax = 0;
-  for (int j = 1 + t; j < a.N; j += blockDim.x) {   // block j is the pivot block of exactly one pair
+  for (int j = 1 + w; j < a.N; j += 4) {   // block j is the pivot block of exactly one pair
     const TW* T = a.Tm + size_t(j) * tsize(n);
-    double pmin = INFINITY, pmax = 0;
-    for (int c = 0; c < n; ++c) {
-      const double d = fabs(double(T[c + size_t(c) * n2]));
-      pmin = fmin(pmin, d);
-      pmax = fmax(pmax, d);
-      for (int r = 0; r <= c; ++r) umax = fmax(umax, fabs(double(T[r + size_t(c) * n2])));
-    }
+    const bool lv = lane < n;
+    const double d = fabs(double(T[(lv ? lane : 0) * (n2 + 1)]));
+    const double pmin = wave_minmax<false>(lv ? d : INFINITY);
+    const double pmax = wave_minmax<true>(lv ? d : 0.0);
+    for (int c = 0; c < n; ++c)
+      if (lane <= c) umax = fmax(umax, fabs(double(T[lane + size_t(c) * n2])));
     const double ratio = pmax > 0 ? pmin / pmax : 0.0;
     dmin = fmin(dmin, pmin);
     dmax = fmax(dmax, pmax);
@@ -155,19 +168,18 @@ __global__ void __launch_bounds__(256) features_batched(const FeatArgs<TW>* args
     rmin = fmin(rmin, ratio);
     cnt += ratio < 1e-5 ? 1.0 : 0.0;
   }
-  r_min[t] = dmin; r_max[t] = dmax; r_sum[t] = lsum; r_rat[t] = rmin; r_cnt[t] = cnt; r_up[t] = umax;
+  umax = wave_minmax<true>(umax);
+  if (lane == 0) { r_min[w] = dmin; r_max[w] = dmax; r_sum[w] = lsum; r_rat[w] = rmin; r_cnt[w] = cnt; r_up[w] = umax; }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      r_min[t] = fmin(r_min[t], r_min[t + w]);
-      r_max[t] = fmax(r_max[t], r_max[t + w]);
-      r_sum[t] += r_sum[t + w];
-      r_rat[t] = fmin(r_rat[t], r_rat[t + w]);
-      r_cnt[t] += r_cnt[t + w];
-      r_up[t] = fmax(r_up[t], r_up[t + w]);
+  if (t == 0)
+    for (int k = 1; k < 4; ++k) {
+      r_min[0] = fmin(r_min[0], r_min[k]);
+      r_max[0] = fmax(r_max[0], r_max[k]);
+      r_sum[0] += r_sum[k];
+      r_rat[0] = fmin(r_rat[0], r_rat[k]);
+      r_cnt[0] += r_cnt[k];
+      r_up[0] = fmax(r_up[0], r_up[k]);
     }
-    __syncthreads();
-  }
   if (t == 0) {
     float* f = out + size_t(a.sys) * 12;
     const int np = a.N - 1;
@@ -280,6 +292,35 @@ hipError_t launch_prep_batched(const PrepArgs<TA, TW>* args, int nsys, hipStream
 template <typename TP, typename TD>
 hipError_t launch_reduce_batched(const ReduceArgs<TP, TD>* args, int nsys, hipStream_t st) {
   hipLaunchKernelGGL((reduce_batched<TP, TD>), dim3(nsys), dim3(256), 0, st, args);
+  return hipGetLastError();
+}
+
+namespace {
+__global__ void __launch_bounds__(256) berr_batched(const NormArgs* args, double* out) {
+  __shared__ double red[256];
+  const NormArgs& a = args[blockIdx.x];
+  if (skipped(a.skip)) return;
+  double m = 0;
+  for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) {
+    const double ar = fabs(a.r[i]), d = a.b[i];
+    const double q = ar == 0.0 ? 0.0 : (d > 0.0 ? ar / d : INFINITY);
+    m = q > m || q != q ? q : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+      const double o = red[threadIdx.x + w];
+      if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[a.sys] = red[0];
+}
+}  // namespace
+
+hipError_t launch_berr_batched(const NormArgs* args, int nsys, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(berr_batched, dim3(nsys), dim3(256), 0, st, args, out);
   return hipGetLastError();
 }
 

@@ -359,13 +359,13 @@ void Engine<TA, TW>::level_factor(Level const& l) {
     hipError_t e = hipSuccess;
     // variants: 0 default (FP32 factors with 16 < n <= 64: the matrix-core
     // pair where it applies; else wave-per-row kernels for n <= 32,
-    // column-split waves up to n = 64, register tiles beyond), 2 two-role (16 < n <= 32),
-    // 3 register tiles only, 4 column-split waves only, 5 matrix-core
-    // right-hand update (FP32 factors, n <= 48)
+    // column-split waves up to n = 64, register tiles beyond), 3 register
+    // tiles only, 4 column-split waves only, 5 matrix-core right-hand update
+    // (FP32 factors, n <= 64)
     const int v = pair_variant_;
     const bool lu_fast =
         method_ == kLU &&
-        ((v == 2 && launch_pair_lu_tw(a, st_, &e)) || (v == 4 && launch_pair_lu_cols(a, st_, &e)) ||
+        ((v == 4 && launch_pair_lu_cols(a, st_, &e)) ||
          (v == 5 && launch_mfma(a, st_, &e)) ||
          (v != 3 && v != 4 && v != 5 &&
           ((d_.n > 16 && launch_mfma(a, st_, &e)) || launch_pair_lu_split(a, st_, &e) ||

@@ -102,9 +102,9 @@ class Engine {
   void set_last2(bool on) { last2_ = on; }
   // LU solves with n <= 32 use the wave-synchronous sweep kernels unless off
   void set_sweep_fast(bool on) { sweep_fast_ = on; }
-  // LU pair kernel choice: 0 default (register kernels for n <= 32, the
-  // register-tiled kernel above), 2 the two-role kernel for 16 < n <= 32,
-  // 3 the register-tiled kernel for every n
+  // LU pair kernel choice: 0 default (see level_factor), 3 the
+  // register-tiled kernel for every n, 4 column-split waves, 5 the
+  // matrix-core pair (FP32 factors)
   void set_pair_variant(int v) { pair_variant_ = v; }
   int method() const { return method_; }
 

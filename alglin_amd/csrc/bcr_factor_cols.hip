@@ -315,17 +315,32 @@ bool launch_pair_lu_cols(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t
   return true;
 }
 
+template <typename TS, typename TW, int NMAX, int NXMAX, int NW, int RS>
+void launch_cols_batched_w(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((pair_lu_cols_batched<TS, TW, NMAX, NXMAX, NW, RS>), dim3(nblocks), dim3(64 * NW), 0, st, t);
+}
+
+// waves per pair: 0 = the default of the element type; 4, 8 or 16 for
+// n <= 48 (two row sets), 8 or 16 beyond (three row sets: four waves would
+// hold 52 columns x 3 rows per lane)
 template <typename TS, typename TW>
 bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
-                                 hipStream_t st, hipError_t* err) {
+                                 hipStream_t st, hipError_t* err, int waves) {
   constexpr int W1 = ColsWaves<TW>::k48, W2 = ColsWaves<TW>::k64;
   switch (cols_variant(n, nr, nx)) {
-    case 1:
-      hipLaunchKernelGGL((pair_lu_cols_batched<TS, TW, 48, 4, W1, 2>), dim3(nblocks), dim3(64 * W1), 0, st, t);
+    case 1: {
+      const int w = waves == 4 || waves == 8 || waves == 16 ? waves : W1;
+      if (w == 4) launch_cols_batched_w<TS, TW, 48, 4, 4, 2>(t, nblocks, st);
+      else if (w == 16) launch_cols_batched_w<TS, TW, 48, 4, 16, 2>(t, nblocks, st);
+      else launch_cols_batched_w<TS, TW, 48, 4, 8, 2>(t, nblocks, st);
       break;
-    case 2:
-      hipLaunchKernelGGL((pair_lu_cols_batched<TS, TW, 64, 16, W2, 3>), dim3(nblocks), dim3(64 * W2), 0, st, t);
+    }
+    case 2: {
+      const int w = waves == 8 || waves == 16 ? waves : W2;
+      if (w == 16) launch_cols_batched_w<TS, TW, 64, 16, 16, 3>(t, nblocks, st);
+      else launch_cols_batched_w<TS, TW, 64, 16, 8, 3>(t, nblocks, st);
       break;
+    }
     default:
       return false;
   }
@@ -336,7 +351,7 @@ bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nb
 #define BCR_INST(TS, TW)                                                                                     \
   template bool launch_pair_lu_cols<TS, TW>(FactorArgs<TS, TW> const&, hipStream_t, hipError_t*);           \
   template bool launch_pair_lu_cols_batched<TS, TW>(BatchTable<FactorArgs<TS, TW>> const&, int, int, int, int, \
-                                                    hipStream_t, hipError_t*);
+                                                    hipStream_t, hipError_t*, int);
 BCR_INST(double, double)
 BCR_INST(float, float)
 BCR_INST(double, float)

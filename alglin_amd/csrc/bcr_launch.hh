@@ -16,8 +16,6 @@ template <typename TS, typename TW>
 bool launch_pair_lu_fast(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 bool launch_pair_lu_split(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
-template <typename TS, typename TW>
-bool launch_pair_lu_tw(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 // register-tiled LU for any n <= 64 (2n + nr <= 160, 3n + nx <= 208)
 bool pair_lu_tiled_fits(int n, int nr, int nx);
 template <typename TS, typename TW>

@@ -190,34 +190,38 @@ __global__ void __launch_bounds__(256) mv_tail_kernel(MvArgs<TA> a, const double
   mv_tail_body<TA, ABS>(a, Csum);
 }
 
-template <typename TA>
+template <typename TA, bool ABS>
 __global__ void __launch_bounds__(256) mv_rows_batched(BatchTable<MvArgs<TA>> t) {
   const int e = t.entry[blockIdx.x];
   if (skipped(t.args[e].skip)) return;
-  mv_rows_body<TA>(t.args[e], int(blockIdx.x) - t.first[e]);
+  mv_rows_body<TA, ABS>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
 // one workgroup per system; Csum[s] = the system's summed border partials
-template <typename TA>
+template <typename TA, bool ABS>
 __global__ void __launch_bounds__(256) mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum) {
   if (skipped(args[blockIdx.x].skip)) return;
-  mv_tail_body<TA>(args[blockIdx.x], Csum[blockIdx.x]);
+  mv_tail_body<TA, ABS>(args[blockIdx.x], Csum[blockIdx.x]);
 }
 
 template <typename TA>
-hipError_t launch_mv_rows_batched(BatchTable<MvArgs<TA>> const& t, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((mv_rows_batched<TA>), dim3(nblocks), dim3(256), 0, st, t);
+hipError_t launch_mv_rows_batched(BatchTable<MvArgs<TA>> const& t, int nblocks, hipStream_t st, bool absval) {
+  if (absval) hipLaunchKernelGGL((mv_rows_batched<TA, true>), dim3(nblocks), dim3(256), 0, st, t);
+  else hipLaunchKernelGGL((mv_rows_batched<TA, false>), dim3(nblocks), dim3(256), 0, st, t);
   return hipGetLastError();
 }
 
 template <typename TA>
-hipError_t launch_mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum, int nsys, hipStream_t st) {
-  hipLaunchKernelGGL((mv_tail_batched<TA>), dim3(nsys), dim3(256), 0, st, args, Csum);
+hipError_t launch_mv_tail_batched(const MvArgs<TA>* args, const double* const* Csum, int nsys, hipStream_t st,
+                                  bool absval) {
+  if (absval) hipLaunchKernelGGL((mv_tail_batched<TA, true>), dim3(nsys), dim3(256), 0, st, args, Csum);
+  else hipLaunchKernelGGL((mv_tail_batched<TA, false>), dim3(nsys), dim3(256), 0, st, args, Csum);
   return hipGetLastError();
 }
 
-template hipError_t launch_mv_rows_batched<double>(BatchTable<MvArgs<double>> const&, int, hipStream_t);
-template hipError_t launch_mv_tail_batched<double>(const MvArgs<double>*, const double* const*, int, hipStream_t);
+template hipError_t launch_mv_rows_batched<double>(BatchTable<MvArgs<double>> const&, int, hipStream_t, bool);
+template hipError_t launch_mv_tail_batched<double>(const MvArgs<double>*, const double* const*, int, hipStream_t,
+                                                   bool);
 
 // sum of squares of v[0..len) -> tmp[blockIdx] (pass 1) ; out = sqrt(sum) (pass 2)
 __global__ void __launch_bounds__(256) sumsq_partial_kernel(const double* __restrict__ v, int64_t len,

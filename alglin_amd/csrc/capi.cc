@@ -768,6 +768,13 @@ int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant) {
   });
 }
 
+int alglin_batch_set_class_kernel(alglin_batch_t h, int fp32_factors, int cls, int kernel, int waves) {
+  return bguard(h, [&] {
+    h->s->set_class_kernel(fp32_factors != 0, cls, kernel, waves);
+    return 0;
+  });
+}
+
 int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods) {
   return bguard(h, [&] {
     h->s->set_methods(std::vector<int32_t>(methods, methods + nsys));

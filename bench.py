@@ -450,6 +450,32 @@ def bench_batched(args):
     print(json.dumps(result))
 
 
+def profile_factorization(args):
+    """The 4096-system uneven batch, FP32 factors: `--steps` eager
+    factorizations with their per-launch HIP-event table (the kernels a
+    counter run of the default workload's dominant class sees, without the
+    rest of the bench), after one 1 GiB copy of known traffic when
+    --calibrate-copy is given."""
+    from alglin_amd import systems
+    from alglin_amd.bcr import Batch, DeviceArray
+    dims = systems.uneven_batch_dims(args.nsys or 4096, args.seed)
+    bt = Batch(mixed=True)
+    bt.plan(dims)
+    bt.generate(seed=args.seed, shift=4.0)
+    if args.calibrate_copy:
+        ca, cb = DeviceArray(1 << 27), DeviceArray(1 << 27)
+        cb.copy_from(ca)
+        ca.free()
+        cb.free()
+    rows = None
+    for _ in range(max(1, args.steps)):
+        rows = bt.time_factorize()
+    cls = rows[:, 1].astype(int)
+    out = {int(c): dict(launches=int((cls == c).sum()), ms=float(rows[cls == c, 3].sum()),
+                        bytes=float(rows[cls == c, 4].sum())) for c in np.unique(cls)}
+    print(json.dumps({"workload": "factor", "nsys": int(len(dims)), "classes": out}))
+
+
 def _single_setup(N, n, qr, nr, shift, seed, kappa=0.0):
     from alglin_amd import systems
     from alglin_amd.bcr import DeviceArray, packed_blocks
@@ -627,9 +653,10 @@ def main():
                     help="fp64: plain FP64 cyclic reduction; mixed: FP32 factors + FP64 refinement to 1e-12")
     ap.add_argument("--kappa", type=float, default=0.0,
                     help="if > 0, D_k = Q diag(s) Q^T with cond(D_k) log-uniform in [1e2, kappa]")
-    ap.add_argument("--workload", choices=["single", "batched", "selector"], default="batched",
+    ap.add_argument("--workload", choices=["single", "batched", "selector", "factor"], default="batched",
                     help="single: one large system; batched: 4096 uneven systems; selector: 8192 systems with "
-                         "the learned LU/QRP choice")
+                         "the learned LU/QRP choice; factor: only the batched FP32-factor factorization of the "
+                         "4096 uneven systems (for counter runs)")
     ap.add_argument("--nsys", type=int, default=0)
     ap.add_argument("--cpu-sample-sys", type=int, default=512)
     ap.add_argument("--no-subs", action="store_true",
@@ -637,7 +664,7 @@ def main():
     ap.add_argument("--per-system-sample", type=int, default=256,
                     help="systems timed one at a time through the plain FP64 BorderedCR (batched workloads)")
     ap.add_argument("--pair-kernel", type=int, default=0,
-                    help="LU pair kernel: 0 default, 2 two-role (16 < n <= 32), 3 register-tiled for every n, "
+                    help="LU pair kernel: 0 default, 3 register-tiled for every n, "
                          "4 column-split waves for every n <= 64, 5 the matrix-core pair (FP32 factors, "
                          "n <= 64); batched workloads: the kernels of the 16 < n <= 64 classes (0 default, "
                          "4 rank-1, 5 matrix-core)")
@@ -649,6 +676,9 @@ def main():
         print("bench.py: this build measures one GPU only (each box has one MI355X); "
               "--gpus above 1 is not supported.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "factor":
+        profile_factorization(args)
+        return
     if args.workload != "single":
         bench_batched(args)
         return

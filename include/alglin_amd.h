@@ -139,8 +139,7 @@ int alglin_bcr_fast_path(alglin_bcr_t h);
 /* LU pair kernel: 0 (default) the matrix-core pair for FP32 factors with
    16 < n <= 64 where it applies (nr <= 8, nx <= 16), else register kernels
    for n <= 32, the column-split-waves kernel up to n = 64 and the
-   register-tiled kernel beyond; 2 the two-role (panel wave + columns wave) kernel for
-   16 < n <= 32; 3 the register-tiled kernel for every n; 4 the
+   register-tiled kernel beyond; 3 the register-tiled kernel for every n; 4 the
    column-split-waves kernel for every n <= 64; 5 the matrix-core pair for
    FP32 factors with n <= 64, nr <= 8, nx <= 16, n + nr <= 48 / 64 / 80 for
    n <= 32 / 48 / 64 (FP64 factors and other shapes: the register-tiled
@@ -246,8 +245,11 @@ int alglin_batch_generate_singular(alglin_batch_t h, uint64_t seed, double shift
    (see alglin_batch_report), 1 on error */
 int alglin_batch_solve(alglin_batch_t h, double const* B, double* X, double tol, int32_t max_iter, int use_graph);
 /* per system: status (bit 0 zero pivot, bit 1 singular last block, bit 2 tol
-   not reached), relative residual, refinement sweeps; summary = {sweeps,
-   max residual, failed systems, wall ms of the last solve}; any may be null */
+   not reached, bit 3 stopped at the FP64 residual floor: the componentwise
+   backward error max_i |b - A x|_i / (|A||x| + |b|)_i is <= 64 u, so no FP64
+   method gets the residual lower and the FP64 fallback is not tried),
+   relative residual, refinement sweeps; summary = {sweeps, max residual,
+   failed systems (status != 0), wall ms of the last solve}; any may be null */
 int alglin_batch_report(alglin_batch_t h, int32_t* status, double* rel_residual, int32_t* sweeps, double summary[4]);
 char const* alglin_batch_last_error(alglin_batch_t h);
 void* alglin_batch_stream(alglin_batch_t h);
@@ -266,6 +268,17 @@ int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows
    for n <= 32 and <= 64 beyond; other shapes take the rank-1 kernels).
    Drops captured graphs.  No counterpart in the reference (a tuning hook). */
 int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant);
+
+/* Tuned kernel of one LU pair-elimination class (no counterpart in the
+   reference: the hook the autotuner, alglin_amd/autotune.py, drives).
+   fp32_factors: 1 the FP32-factor engine of a mixed batch, 0 the FP64 one
+   (plain batches, and the FP64 fallback of a mixed one).  cls: 3 for
+   16 < n <= 32, 4 for 32 < n <= 64.  kernel: -1 default, 0 the matrix-core
+   pair (FP32 factors), 1 the rank-1 kernels (two-wave split for class 3,
+   column-split for class 4 with `waves` waves per pair: 4, 8 or 16 for
+   n <= 48, 8 or 16 beyond, 0 = default), 2 the register-tiled kernel
+   (class 4).  Drops captured graphs. */
+int alglin_batch_set_class_kernel(alglin_batch_t h, int fp32_factors, int cls, int kernel, int waves);
 
 /* per-system pair elimination method for alglin_batch_solve (0 LU, 1 QR, 2 QRP) */
 int alglin_batch_set_methods(alglin_batch_t h, int32_t nsys, int32_t const* methods);

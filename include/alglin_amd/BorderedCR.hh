@@ -300,6 +300,7 @@ class BorderedCR {
   void fail(std::string const& msg) const;
   void check_ld(char const* what, integer nbl, integer ld, integer rows) const;
   void check_dims(char const* what, MatW const& M, integer nr, integer nc) const;
+  void check_supported() const;   // n <= 64, nr <= 16
   integer d_N_ = 0, d_n_ = 0, d_qr_ = 0, d_qx_ = 0, d_nr_ = 0, d_nx_ = 0;
   integer max_pb_ = 1;
   BORDERED_Choice sel_ = BORDERED_Choice::LU;

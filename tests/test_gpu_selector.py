@@ -106,3 +106,62 @@ def test_committed_selector_cuts_failures_without_fallback():
     assert sel["failed"] < base["failed"]
     ok = sel["status"] == 0
     assert np.all(sel["rel_residual"][ok] <= 1e-12)
+
+
+def test_selector_workload_full_size():
+    """The learned-selector workload at full size: 8192 systems, n in
+    {8, 16, 32}, N in [64, 2048], half with a rank-deficient pair column
+    (1e-10), mixed + committed selector + FP64 fallback.  Size-independent
+    properties: every system reported solved has a residual <= 1e-12, every
+    other one is marked (a system that stopped at the FP64 floor carries
+    status bit 3 and never went through the FP64 fallback), and the systems
+    the FP64 path took are mostly the nearly singular ones."""
+    from alglin_amd.bcr import DeviceArray
+    mlp = S.load()
+    bt, B, sing = S.make_batch(8192, seed=2024, Nmin=64, Nmax=2048)
+    X = DeviceArray(bt.size("x"))
+    bt.solve_selected(B, X, tol=1e-12, selector=mlp, fallback=True)
+    rep = bt.report()
+    sel = bt.selection()
+    st = rep["status"]
+    print(f"failed {rep['failed']}, flagged {int(sel['chose_qrp'].sum())}, FP64 path {int(sel['fell_back'].sum())}, "
+          f"floor {int(((st & 8) != 0).sum())}, max sweeps {rep['max_sweeps']}", flush=True)
+    ok = st == 0
+    assert np.all(rep["rel_residual"][ok] <= 1e-12)
+    floor = (st & 8) != 0
+    assert not np.any(sel["fell_back"][floor])
+    assert rep["failed"] <= 8
+    fb = sel["fell_back"]
+    assert (fb & (sing >= 1)).sum() > 4 * (fb & (sing < 1)).sum()
+
+
+def test_default_batch_floor_systems_skip_the_fallback():
+    """The 4096-system uneven batch of the default bench workload has systems
+    whose FP64 residual floor lies above 1e-12 (plain FP64 misses it too).
+    They end with status bit 3 in plain FP64 and in the learned path, and the
+    learned path does not send them through the FP64 fallback."""
+    from alglin_amd import systems
+    from alglin_amd.bcr import Batch, DeviceArray
+    dims = systems.uneven_batch_dims(4096, 2024)
+    ref = Batch(mixed=False)
+    ref.plan(dims)
+    ref.generate(seed=2024, shift=4.0)
+    mix = Batch(mixed=True)
+    mix.plan(dims, operator=False)
+    mix.share_operator(ref)
+    L = ref.size("x")
+    B = DeviceArray.from_host(np.random.default_rng(2029).standard_normal(L))
+    X = DeviceArray(L)
+    ref.solve(B, X, 1e-12, 30, graph=True)
+    r64 = ref.report()
+    mix.solve_selected(B, X, 1e-12, 30, selector=S.load(), fallback=True)
+    rm = mix.report()
+    sel = mix.selection()
+    f64 = (r64["status"] & 8) != 0
+    fm = (rm["status"] & 8) != 0
+    print(f"FP64: failed {r64['failed']} floor {int(f64.sum())}; learned: failed {rm['failed']} floor {int(fm.sum())} "
+          f"FP64 path {int(sel['fell_back'].sum())}", flush=True)
+    assert np.all((r64["status"] == 0) | f64)
+    assert np.all((rm["status"] == 0) | fm)
+    assert not np.any(sel["fell_back"][fm])
+    assert np.all(rm["rel_residual"][rm["status"] == 0] <= 1e-12)
