@@ -14,15 +14,14 @@
 namespace bcr {
 namespace {
 
-// the matrix-core pair of the 16 < n <= 32 and 32 < n <= 48 LU classes
-// (FP32 factors only): the default (variant 0) and variant 5; variant 4
-// keeps the rank-1 kernels (two-wave split, column-split)
+// the matrix-core pair of the 16 < n <= 32 and 32 < n <= 48 LU classes:
+// FP32 factors by default (variant 0) and with variant 5, FP64 factors with
+// variant 5 (a tuned class choice); variant 4 keeps the rank-1 kernels
+// (two-wave split, column-split)
 template <typename TS, typename TW, typename L>
 bool try_mfma(int variant, BatchTable<FactorArgs<TS, TW>> const& t, L const& l, hipStream_t st, hipError_t* e) {
-  if constexpr (std::is_same_v<TW, float>) {
-    if (variant == 0 || variant == 5) return launch_pair_lu_mfma_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st, e);
-  }
-  return false;
+  const bool want = variant == 5 || (variant == 0 && std::is_same_v<TW, float>);
+  return want && launch_pair_lu_mfma_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st, e);
 }
 
 // host staging of the launch tables; uploaded once as one device blob
@@ -194,6 +193,7 @@ void BatchEngine<TA, TW>::invalidate() {
   mv_cache_.clear();
   norm_cache_.clear();
   add_cache_.clear();
+  feat_ok_ = false;
 }
 
 template <typename TA, typename TW>
@@ -221,6 +221,7 @@ void BatchEngine<TA, TW>::swap_stash() {
   stash_.mv.swap(mv_cache_);
   stash_.norm.swap(norm_cache_);
   stash_.add.swap(add_cache_);
+  feat_ok_ = false;   // built for the active set of its call
 }
 
 template <typename TA, typename TW>
@@ -779,21 +780,39 @@ void BatchEngine<TA, TW>::norms_async(double const* B, double const* R, double* 
 template <typename TA, typename TW>
 void BatchEngine<TA, TW>::features_async(double const* norms, float* out) {
   if (!tables_ok_) build_factor_tables();
-  std::vector<FeatArgs<TW>> f(active_.size());
-  for (size_t i = 0; i < active_.size(); ++i) {
-    const int s = active_[i];
-    SysDims const& d = dims_[s];
-    FeatArgs<TW>& a = f[i];
-    a.N = d.N; a.n = d.n; a.qr = d.qr; a.nr = d.nr; a.Nc = d.Nc(); a.sys = s;
-    a.Tm = Tm_.p + offT_[s]; a.Hm = Hm_.p + offHm_[s]; a.norms = norms;
+  if (active_.empty()) return;
+  if (!feat_ok_ || feat_norms_ != norms) {
+    // panels j = 1 .. N-1 of the active systems, in order, four per workgroup
+    std::vector<FeatArgs<TW>> f(active_.size());
+    std::vector<int32_t> entry, first(active_.size());
+    std::vector<size_t> pan(active_.size());
+    size_t np = 0;
+    for (size_t i = 0; i < active_.size(); ++i) {
+      const int s = active_[i];
+      SysDims const& d = dims_[s];
+      FeatArgs<TW>& a = f[i];
+      a.N = d.N; a.n = d.n; a.qr = d.qr; a.nr = d.nr; a.Nc = d.Nc(); a.sys = s;
+      a.Tm = Tm_.p + offT_[s]; a.Hm = Hm_.p + offHm_[s]; a.norms = norms;
+      pan[i] = np;
+      np += size_t(std::max(d.N - 1, 0));
+      first[i] = int32_t(entry.size());
+      for (int k = 0; k < (d.N - 1 + 3) / 4; ++k) entry.push_back(int32_t(i));
+    }
+    pstat_.resize(3 * std::max<size_t>(np, 1));
+    for (size_t i = 0; i < f.size(); ++i) f[i].pstat = pstat_.p + 3 * pan[i];
+    Blob bb;
+    const size_t oa = bb.add(f), oe = bb.add(entry), of = bb.add(first);
+    bb.upload(feat_blob_, st_);
+    feat_args_ = reinterpret_cast<const FeatArgs<TW>*>(feat_blob_.p + oa);
+    feat_tab_ = BatchTable<FeatArgs<TW>>{feat_args_, reinterpret_cast<const int32_t*>(feat_blob_.p + oe),
+                                         reinterpret_cast<const int32_t*>(feat_blob_.p + of)};
+    feat_wgs_ = int(entry.size());
+    feat_nsys_ = int(f.size());
+    feat_norms_ = norms;
+    feat_ok_ = true;
   }
-  if (f.empty()) return;
-  Blob bb;
-  const size_t o = bb.add(f);
-  bb.upload(scratch_, st_);
-  hip_check(launch_features_batched<TW>(reinterpret_cast<const FeatArgs<TW>*>(scratch_.p + o), int(f.size()), out, st_),
-            "features");
-  hip_check(hipStreamSynchronize(st_), "features sync");
+  hip_check(launch_panel_stats_batched<TW>(feat_tab_, feat_wgs_, st_), "panel statistics");
+  hip_check(launch_features_batched<TW>(feat_args_, feat_nsys_, out, st_), "features");
 }
 
 template <typename TA, typename TW>
@@ -1159,7 +1178,9 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
   std::vector<char> direct(ns, 0);   // flagged systems sent straight to the FP64 path
   dfeat_.resize(size_t(ns) * 12);
   eng.features_async(norms_.p, dfeat_.p);
-  hip_check(hipMemcpy(feat_.data(), dfeat_.p, feat_.size() * sizeof(float), hipMemcpyDeviceToHost), "features");
+  hip_check(hipMemcpyAsync(feat_.data(), dfeat_.p, feat_.size() * sizeof(float), hipMemcpyDeviceToHost, st),
+            "features");
+  hip_check(hipStreamSynchronize(st), "features sync");
   if (selector) {
     dprob_.resize(size_t(ns));
     selector->predict(dfeat_.p, ns, dprob_.p);

@@ -216,6 +216,15 @@ class BatchEngine {
   std::map<std::vector<const void*>, MvTables> mv_cache_;
   std::map<std::vector<const void*>, NormTables> norm_cache_;
   std::map<std::vector<const void*>, AddTables> add_cache_;
+  // the feature launches' records and tables (active set and norms pointer
+  // of the call that built them), and the per-panel statistics
+  DevBuf<char> feat_blob_;
+  DevBuf<float> pstat_;
+  const FeatArgs<TW>* feat_args_ = nullptr;
+  BatchTable<FeatArgs<TW>> feat_tab_{};
+  int feat_wgs_ = 0, feat_nsys_ = 0;
+  const double* feat_norms_ = nullptr;
+  bool feat_ok_ = false;
 
   // the tables set aside by narrow()
   struct Stash {

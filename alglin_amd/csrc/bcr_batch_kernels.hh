@@ -44,13 +44,17 @@ struct GenArgs {
 
 // per-system features of the LU-vs-QRP selector (12 floats at out[12*sys]),
 // from the factored pair panels, the factored last block and the residual
-// of the first solve
+// of the first solve.  Two launches: panel_stats (one wave per pair panel,
+// every system at once) writes (min |u_ii|, max |u_ii|, max_{i<=k} |u_ik|)
+// of each panel to pstat; features (one workgroup per system) reduces its
+// panels' triples and adds the last-block and residual terms.
 template <typename TW>
 struct FeatArgs {
   int N, n, qr, nr, Nc, sys;
   const TW* Tm;          // N panels of tsize(n)
   const double* Hm;      // factored Nc x Nc last block
   const double* norms;   // norms[2*sys] = ||b||, norms[2*sys+1] = ||r||
+  float* pstat;          // 3 floats per panel j = 1 .. N-1 of this system
 };
 
 struct CopyArgs {
@@ -60,6 +64,10 @@ struct CopyArgs {
   const int32_t* skip = nullptr;   // the system is done when *skip != 0
 };
 
+// panels: BatchTable over workgroups of four panels (workgroup b of record e
+// covers panels j = 1 + 4 (b - first[e]) + w, w < 4)
+template <typename TW>
+hipError_t launch_panel_stats_batched(BatchTable<FeatArgs<TW>> const& t, int nblocks, hipStream_t st);
 template <typename TW>
 hipError_t launch_features_batched(const FeatArgs<TW>* args, int nsys, float* out, hipStream_t st);
 hipError_t launch_copy_batched(const CopyArgs* args, int nsys, hipStream_t st);
@@ -120,8 +128,8 @@ bool launch_pair_lu_tiled_batched(BatchTable<FactorArgs<TS, TW>> const& t, int n
 template <typename TS, typename TW>
 bool launch_pair_lu_cols_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err, int waves = 0);
-template <typename TS>
-bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
+template <typename TS, typename TW>
+bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 hipError_t launch_pair_generic_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, size_t bytes,

@@ -8,7 +8,11 @@
 //   norms_batched     out[2s] = ||b_s||_2, out[2s+1] = ||r_s||_2
 //   gen_batched       counter-based Gaussian blocks (splitmix64 hash of
 //                     (seed, system, kind, index), Box-Muller)
+//   panel_stats_batched, features_batched
+//                     the selector's features: per-panel pivot statistics
+//                     (one wave per panel), then one workgroup per system
 #include "bcr_batch_kernels.hh"
+#include "bcr_wave.hh"
 
 namespace bcr {
 namespace {
@@ -132,55 +136,92 @@ __global__ void __launch_bounds__(256) gen_batched(const GenArgs* args, int kind
   }
 }
 
-template <bool MAX>
-__device__ __forceinline__ double wave_minmax(double v) {
+// one wave per pair panel (four per workgroup, every system in one launch):
+// the lanes walk the top n x n block of T (U on and above the diagonal) in
+// flat order, four loads in flight per lane, and write the panel's min and
+// max pivot magnitude and its largest |u_ik| (i <= k); the wave reductions
+// run on the float bit patterns (non-negative floats order as unsigned)
+template <typename TW>
+__global__ void __launch_bounds__(256) panel_stats_batched(BatchTable<FeatArgs<TW>> t) {
+  const int e = t.entry[blockIdx.x];
+  const FeatArgs<TW>& a = t.args[e];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int j = 1 + 4 * (int(blockIdx.x) - t.first[e]) + w;
+  if (j >= a.N) return;   // no barrier in this kernel
+  const int n = a.n, n2 = 2 * n, nn = n * n;
+  const TW* T = a.Tm + size_t(j) * tsize(n);
+  const int dr = 64 % n, dc = 64 / n;
+  int row = lane % n, col = lane / n;
+  float dmin = INFINITY, dmax = 0.f, umax = 0.f;
+  for (int k0 = 0; k0 < nn; k0 += 256) {
+    float v[4];
+    int rr[4], cc[4];
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const double o = __shfl_xor(v, off);
-    v = MAX ? fmax(v, o) : fmin(v, o);
+    for (int u = 0; u < 4; ++u) {
+      rr[u] = row;
+      cc[u] = col;
+      v[u] = keep_if(float(fabs(T[row + size_t(col < n ? col : n - 1) * n2])), col < n);
+      row += dr;
+      col += dc;
+      if (row >= n) {
+        row -= n;
+        ++col;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (cc[u] < n && rr[u] == cc[u]) {
+        dmin = fminf(dmin, v[u]);
+        dmax = fmaxf(dmax, v[u]);
+      }
+      if (cc[u] < n && rr[u] <= cc[u]) umax = fmaxf(umax, v[u]);
+    }
   }
-  return v;
+  const unsigned pmin = wave::min_u32(__float_as_uint(dmin));
+  const unsigned pmax = wave::max_u32(__float_as_uint(dmax));
+  const unsigned um = wave::max_u32(__float_as_uint(umax));
+  if (lane == 0) {
+    float* o = a.pstat + 3 * size_t(j - 1);
+    o[0] = __uint_as_float(pmin);
+    o[1] = __uint_as_float(pmax);
+    o[2] = __uint_as_float(um);
+  }
 }
 
-// one workgroup per system, its four waves taking the pair panels in turn:
-// per panel the lanes read the pivots (the diagonal of U) and, column by
-// column, the upper triangle (coalesced within a column), so a panel costs
-// about n loads per lane instead of n(n+1)/2 loads of one thread
+// one workgroup per system: the panels' triples (coalesced), a block
+// reduction, and the 12 features
 template <typename TW>
 __global__ void __launch_bounds__(256) features_batched(const FeatArgs<TW>* args, float* out) {
-  __shared__ double r_min[4], r_max[4], r_sum[4], r_rat[4], r_cnt[4], r_up[4];
+  __shared__ double r_min[256], r_max[256], r_sum[256], r_rat[256], r_cnt[256], r_up[256];
   const FeatArgs<TW>& a = args[blockIdx.x];
-  const int n = a.n, n2 = 2 * n, t = threadIdx.x, lane = t & 63, w = t >> 6;
-  // wave-uniform accumulators over this wave's panels; umax per lane
+  const int t = threadIdx.x;
   double dmin = INFINITY, dmax = 0, lsum = 0, rmin = INFINITY, cnt = 0, umax = 0;
-  for (int j = 1 + w; j < a.N; j += 4) {   // block j is the pivot block of exactly one pair
-    const TW* T = a.Tm + size_t(j) * tsize(n);
-    const bool lv = lane < n;
-    const double d = fabs(double(T[(lv ? lane : 0) * (n2 + 1)]));
-    const double pmin = wave_minmax<false>(lv ? d : INFINITY);
-    const double pmax = wave_minmax<true>(lv ? d : 0.0);
-    for (int c = 0; c < n; ++c)
-      if (lane <= c) umax = fmax(umax, fabs(double(T[lane + size_t(c) * n2])));
+  for (int i = t; i < a.N - 1; i += 256) {
+    const float* ps = a.pstat + 3 * size_t(i);
+    const double pmin = ps[0], pmax = ps[1], pu = ps[2];
     const double ratio = pmax > 0 ? pmin / pmax : 0.0;
     dmin = fmin(dmin, pmin);
     dmax = fmax(dmax, pmax);
     lsum += log10(ratio + 1e-30);
     rmin = fmin(rmin, ratio);
     cnt += ratio < 1e-5 ? 1.0 : 0.0;
+    umax = fmax(umax, pu);
   }
-  umax = wave_minmax<true>(umax);
-  if (lane == 0) { r_min[w] = dmin; r_max[w] = dmax; r_sum[w] = lsum; r_rat[w] = rmin; r_cnt[w] = cnt; r_up[w] = umax; }
+  r_min[t] = dmin; r_max[t] = dmax; r_sum[t] = lsum; r_rat[t] = rmin; r_cnt[t] = cnt; r_up[t] = umax;
   __syncthreads();
-  if (t == 0)
-    for (int k = 1; k < 4; ++k) {
-      r_min[0] = fmin(r_min[0], r_min[k]);
-      r_max[0] = fmax(r_max[0], r_max[k]);
-      r_sum[0] += r_sum[k];
-      r_rat[0] = fmin(r_rat[0], r_rat[k]);
-      r_cnt[0] += r_cnt[k];
-      r_up[0] = fmax(r_up[0], r_up[k]);
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      r_min[t] = fmin(r_min[t], r_min[t + h]);
+      r_max[t] = fmax(r_max[t], r_max[t + h]);
+      r_sum[t] += r_sum[t + h];
+      r_rat[t] = fmin(r_rat[t], r_rat[t + h]);
+      r_cnt[t] += r_cnt[t + h];
+      r_up[t] = fmax(r_up[t], r_up[t + h]);
     }
+    __syncthreads();
+  }
   if (t == 0) {
+    const int n = a.n;
     float* f = out + size_t(a.sys) * 12;
     const int np = a.N - 1;
     f[0] = float(log2(double(a.N)) / 12.0);
@@ -259,6 +300,13 @@ __global__ void __launch_bounds__(256) add_batched(const CopyArgs* args) {
 }  // namespace
 
 template <typename TW>
+hipError_t launch_panel_stats_batched(BatchTable<FeatArgs<TW>> const& t, int nblocks, hipStream_t st) {
+  if (nblocks < 1) return hipSuccess;
+  hipLaunchKernelGGL((panel_stats_batched<TW>), dim3(nblocks), dim3(256), 0, st, t);
+  return hipGetLastError();
+}
+
+template <typename TW>
 hipError_t launch_features_batched(const FeatArgs<TW>* args, int nsys, float* out, hipStream_t st) {
   hipLaunchKernelGGL((features_batched<TW>), dim3(nsys), dim3(256), 0, st, args, out);
   return hipGetLastError();
@@ -280,6 +328,8 @@ hipError_t launch_converge_batched(ConvergeArgs const& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+template hipError_t launch_panel_stats_batched<float>(BatchTable<FeatArgs<float>> const&, int, hipStream_t);
+template hipError_t launch_panel_stats_batched<double>(BatchTable<FeatArgs<double>> const&, int, hipStream_t);
 template hipError_t launch_features_batched<float>(const FeatArgs<float>*, int, float*, hipStream_t);
 template hipError_t launch_features_batched<double>(const FeatArgs<double>*, int, float*, hipStream_t);
 

@@ -165,3 +165,61 @@ def test_default_batch_floor_systems_skip_the_fallback():
     assert np.all((rm["status"] == 0) | fm)
     assert not np.any(sel["fell_back"][fm])
     assert np.all(rm["rel_residual"][rm["status"] == 0] <= 1e-12)
+
+
+def host_panel_features(T, N, n):
+    """Features 4..8 and 11 from FP32 LU factors, by their definition: per
+    pivot block j = 1..N-1 the pivots u_ii of the panel's U (the top n x n
+    block of its 2n x n factor, column-major) give min/max |u_ii| and their
+    ratio; the features are log10 of the smallest and largest pivot, the mean
+    log10 ratio, log10 of the smallest ratio, the share of ratios below 1e-5,
+    and log10 of max |u_ik| (i <= k) over the largest pivot."""
+    ts = 2 * n * n + n
+    pmin, pmax, umax = [], [], 0.0
+    for j in range(1, N):
+        P = T[j * ts:j * ts + 2 * n * n].astype(np.float64).reshape(n, 2 * n).T   # row r, column c
+        U = np.abs(np.triu(P[:n, :]))
+        d = np.abs(np.diag(P[:n, :]))
+        pmin.append(d.min())
+        pmax.append(d.max())
+        umax = max(umax, U.max())
+    pmin, pmax = np.array(pmin), np.array(pmax)
+    ratio = np.where(pmax > 0, pmin / np.where(pmax > 0, pmax, 1), 0.0)
+    return np.array([np.log10(pmin.min() + 1e-30), np.log10(pmax.max() + 1e-30), np.mean(np.log10(ratio + 1e-30)),
+                     np.log10(ratio.min() + 1e-30), np.mean(ratio < 1e-5),
+                     np.log10(umax / (pmax.max() + 1e-30) + 1e-30)])
+
+
+def test_batch_features_match_the_factors():
+    """The batch's selector features (per-panel statistics kernel, then one
+    workgroup per system) against the same statistics computed on the host
+    from the single-system FP32 LU factors of each system (the same kernels:
+    the pivots and U agree bit for bit), and the shape features 0..3."""
+    from alglin_amd import systems
+    from alglin_amd.bcr import Batch, BorderedCR, DeviceArray
+    dims = np.asarray([(7, 5, 1, 1, 2, 2, 0), (33, 20, 5, 5, 2, 2, 0), (2, 40, 10, 10, 2, 2, 0),
+                       (1, 8, 2, 2, 2, 2, 0), (130, 48, 12, 12, 2, 2, 0), (64, 16, 4, 4, 3, 3, 0),
+                       (9, 1, 0, 0, 1, 1, 0)], np.int32)
+    print(f"case features nsys={len(dims)}", flush=True)
+    bt = Batch(mixed=True)
+    bt.plan(dims)
+    bt.generate(seed=11, shift=4.0)
+    B = DeviceArray.from_host(np.random.default_rng(3).standard_normal(bt.size("x")))
+    X = DeviceArray(bt.size("x"))
+    bt.solve_selected(B, X, tol=1e-12, selector=None, fallback=False)
+    f = bt.selection()["features"]
+    for s, (N, n, qr, qx, nr, nx, _) in enumerate(dims):
+        assert f[s, 0] == pytest.approx(np.log2(N) / 12, rel=1e-6)
+        assert f[s, 1] == pytest.approx(n / 48, rel=1e-6)
+        assert f[s, 2] == pytest.approx(qr / 12, rel=1e-6)
+        assert f[s, 3] == pytest.approx(nr / 8, rel=1e-6)
+        if N < 2:
+            assert np.all(f[s, [4, 5, 6, 7, 8, 11]] == 0)
+            continue
+        g = BorderedCR(np.float32)
+        g.load_system(bt.system(s))
+        g.select("LU", "LU", "NONE")
+        assert g.factorize(), g.last_error()
+        want = host_panel_features(g.factors()["T"], int(N), int(n))
+        got = f[s, [4, 5, 6, 7, 8, 11]].astype(np.float64)
+        assert np.allclose(got, want, rtol=1e-5, atol=1e-5), (s, got, want)
