@@ -14,8 +14,9 @@ Candidates (alglin_batch_set_class_kernel codes):
   class 3, FP32 factors: the matrix-core pair (0), the two-wave split (1)
   class 4, FP32 factors: the matrix-core pair (0), column-split waves with
                          4 / 8 / 16 waves per pair (1), register tiles (2)
-  class 4, FP64 factors: column-split waves 4 / 8 / 16 (1), register tiles (2)
-(class 3 with FP64 factors has one kernel, the two-wave split.)
+  class 3, FP64 factors: the matrix-core pair (0), the two-wave split (1)
+  class 4, FP64 factors: the matrix-core pair (0, n <= 48), column-split
+                         waves 4 / 8 / 16 (1), register tiles (2)
 
 Usage: python -m alglin_amd.autotune [--nsys 1024] [--seed 3] [--reps 3] [--out alglin_amd/tuning.json]
 """
@@ -35,7 +36,8 @@ from .bcr import TUNING_PATH, Batch, device_info
 CANDIDATES = {
     (True, 3): [(0, 0), (1, 0)],
     (True, 4): [(0, 0), (1, 4), (1, 8), (1, 16), (2, 0)],
-    (False, 4): [(1, 4), (1, 8), (1, 16), (2, 0)],
+    (False, 3): [(0, 0), (1, 0)],
+    (False, 4): [(0, 0), (1, 4), (1, 8), (1, 16), (2, 0)],
 }
 
 

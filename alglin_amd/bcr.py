@@ -620,15 +620,16 @@ class Batch:
 
     def set_pair_kernel(self, variant: int):
         """Kernels of the 16 < n <= 32 and 32 < n <= 64 LU pair classes: 0 default
-        (the matrix-core pair where it applies, else the rank-1 kernels), 4 the
-        rank-1 kernels (two-wave split, column-split), 5 the matrix-core pair
-        (FP32 factors, n <= 48, nr <= 8)."""
+        (the matrix-core pair where it applies: n <= 48, nr <= 8, nx <= 16;
+        else the rank-1 kernels), 4 the rank-1 kernels (two-wave split,
+        column-split), 5 the matrix-core pair."""
         self._check(lib().alglin_batch_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def set_class_kernel(self, cls: int, kernel: int, waves: int = 0, fp32_factors=None):
         """Kernel of LU pair class cls (3: 16 < n <= 32, 4: 32 < n <= 64):
-        -1 default, 0 matrix-core pair, 1 rank-1 (column-split with `waves`
-        waves per pair for class 4), 2 register-tiled (class 4)."""
+        -1 default, 0 matrix-core pair (FP32 or FP64 factors, n <= 48), 1
+        rank-1 (column-split with `waves` waves per pair for class 4), 2
+        register-tiled (class 4)."""
         fp32 = self.mixed if fp32_factors is None else bool(fp32_factors)
         self._check(lib().alglin_batch_set_class_kernel(self._h, int(fp32), cls, kernel, waves), "set_class_kernel")
 

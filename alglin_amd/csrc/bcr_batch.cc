@@ -14,14 +14,13 @@
 namespace bcr {
 namespace {
 
-// the matrix-core pair of the 16 < n <= 32 and 32 < n <= 48 LU classes:
-// FP32 factors by default (variant 0) and with variant 5, FP64 factors with
-// variant 5 (a tuned class choice); variant 4 keeps the rank-1 kernels
-// (two-wave split, column-split)
+// the matrix-core pair of the 16 < n <= 32 and 32 < n <= 48 LU classes
+// (FP64 factors: n <= 48; FP32 factors also up to 64): the default (variant
+// 0) and variant 5; variant 4 keeps the rank-1 kernels (two-wave split,
+// column-split)
 template <typename TS, typename TW, typename L>
 bool try_mfma(int variant, BatchTable<FactorArgs<TS, TW>> const& t, L const& l, hipStream_t st, hipError_t* e) {
-  const bool want = variant == 5 || (variant == 0 && std::is_same_v<TW, float>);
-  return want && launch_pair_lu_mfma_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st, e);
+  return (variant == 0 || variant == 5) && launch_pair_lu_mfma_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, st, e);
 }
 
 // host staging of the launch tables; uploaded once as one device blob
@@ -107,14 +106,18 @@ void BatchEngine<TA, TW>::join() {
   hip_check(hipStreamWaitEvent(st_, join_, 0), "join");
 }
 
-// the launch of a system's class takes a panel in LDS only when no register
-// kernel covers the shape: LU with n <= 32 (wave kernels), LU up to n = 64
-// through the column-split or register-tiled kernels; QR and QRP always
+// the launch of a system's class takes a whole panel in LDS only when no
+// other kernel covers the shape: LU with n <= 32 (wave kernels), LU up to
+// n = 64 through the column-split or register-tiled kernels; QR and QRP
+// through the chunked QR kernel (T and a column chunk in LDS)
 template <typename TA, typename TW>
 bool BatchEngine<TA, TW>::panel_fits(SysDims const& d, size_t lds_limit) {
   const int c = factor_class(d);
   if (c < 4) return true;
   if (c == 4 && (pair_lu_cols_fits(d.n, d.nr, d.nx) || pair_lu_tiled_fits(d.n, d.nr, d.nx))) return true;
+  int cw = 0;
+  size_t bytes = 0;
+  if (d.method != kLU && pair_qr_plan(d.n, d.nr, d.nx, sizeof(TW), lds_limit, &cw, &bytes)) return true;
   return size_t(2 * d.n + d.nr) * (3 * d.n + d.nx) * sizeof(TW) <= lds_limit;
 }
 
@@ -477,6 +480,9 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
           ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e, cls_waves_[l.cls]);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
           ok = launch_pair_lu_tiled_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
+        else if (l.method != kLU &&
+                 launch_pair_qr_batched(l.method, t, l.nblocks, l.nmax, l.nrmax, l.nxmax, lds_limit_, s, &e))
+          ok = true;
         else {
           const size_t bytes = size_t(2 * l.nmax + l.nrmax) * (3 * l.nmax + l.nxmax) * sizeof(TW);
           e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, s);

@@ -122,7 +122,7 @@ class BatchEngine {
   void narrow(std::vector<char> const& mask);
   void widen();
   // kernels of the 16 < n <= 32 and 32 < n <= 64 LU classes: 0 default (the
-  // matrix-core pair where it applies: FP32 factors, n <= 48; else the
+  // matrix-core pair where it applies: n <= 48, nr <= 8, nx <= 16; else the
   // rank-1 kernels), 4 the rank-1 kernels (two-wave split, column-split),
   // 5 the matrix-core pair
   void set_pair_variant(int v) { pair_variant_ = v; }

@@ -132,6 +132,9 @@ template <typename TS, typename TW>
 bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
+bool launch_pair_qr_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nr, int nx,
+                            size_t lds_limit, hipStream_t st, hipError_t* err);
+template <typename TS, typename TW>
 hipError_t launch_pair_generic_batched(int method, BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, size_t bytes,
                                        size_t lds_limit, hipStream_t st);
 

@@ -11,11 +11,10 @@
 namespace bcr {
 
 namespace {
-// the matrix-core pair kernel exists for FP32 factors only
+// the matrix-core pair (FP32 factors n <= 64, FP64 factors n <= 48)
 template <typename TS, typename TW>
 bool launch_mfma(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* e) {
-  if constexpr (std::is_same_v<TW, float>) return launch_pair_lu_mfma(a, st, e);
-  return false;
+  return launch_pair_lu_mfma(a, st, e);
 }
 }  // namespace
 
@@ -357,11 +356,11 @@ void Engine<TA, TW>::level_factor(Level const& l) {
     using A = std::remove_reference_t<decltype(a)>;
     (void)sizeof(A);
     hipError_t e = hipSuccess;
-    // variants: 0 default (FP32 factors with 16 < n <= 64: the matrix-core
-    // pair where it applies; else wave-per-row kernels for n <= 32,
-    // column-split waves up to n = 64, register tiles beyond), 3 register
-    // tiles only, 4 column-split waves only, 5 matrix-core right-hand update
-    // (FP32 factors, n <= 64)
+    // variants: 0 default (16 < n: the matrix-core pair where it applies,
+    // FP32 factors n <= 64, FP64 factors n <= 48; else wave-per-row kernels
+    // for n <= 32, column-split waves up to n = 64, register tiles beyond), 3
+    // register tiles only, 4 column-split waves only, 5 the matrix-core pair
+    // where it applies
     const int v = pair_variant_;
     const bool lu_fast =
         method_ == kLU &&
@@ -374,6 +373,10 @@ void Engine<TA, TW>::level_factor(Level const& l) {
     if (lu_fast) {
       stats_.fast_path = 1;
       hip_check(e, "pair_lu_fast");
+      return;
+    }
+    if (method_ != kLU && launch_pair_qr(method_, a, lds_limit_, st_, &e)) {
+      hip_check(e, "pair_qr");
       return;
     }
     const size_t bytes = size_t(2 * d_.n + d_.nr) * (3 * d_.n + d_.nx) * sizeof(TW);

@@ -1,4 +1,4 @@
-// Pair elimination, LU, FP32 factors, n <= 48: the getrf + applyT split of
+// Pair elimination, LU, FP32 factors n <= 64 and FP64 factors n <= 48: the getrf + applyT split of
 // the reference pair step (buildT BABD_BorderedCR.cc:856, applyT cc:917-982,
 // the border update cc:1107-1154), with the right-hand columns on the matrix
 // cores.  The augmented panel is the one of bcr_factor_generic.hip: R = 2n +
@@ -27,10 +27,12 @@
 //                                           -> pivot rows: D_j', E_j', B_j'
 //        X_rest' = X_rest - [L21; C_j U^-1] U12
 //                                           -> reduced rows D_jp', E_jp', B_jp'
-//     on v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation).  The sum
-//     over k runs as 16-wide blocks, lane group g = lane / 16 taking k = 16 kb
-//     + 4 g + i in the i-th MFMA of a block: every LDS operand is one 16-byte
-//     read per lane per four MFMAs, and a block of U12 in the accumulator
+//     on v_mfma_f32_16x16x4f32 (FP32 factors) or v_mfma_f64_16x16x4f64 (FP64
+//     factors).  The sum over k runs as 16-wide blocks, lane group
+//     g = lane / 16 taking k = 16 kb + kro(g, i) in the i-th MFMA of a block
+//     (kro: the row the result layout puts in value i of group g, 4 g + i in
+//     FP32, g + 4 i in FP64): in FP32 every LDS operand is one 16-byte read
+//     per lane per four MFMAs, and in both a block of U12 in the accumulator
 //     layout is the next products' B operand as it stands.  The border rows
 //     (C_jp', pending C_jpp, F partial) leave through LDS in one coalesced
 //     pass per tile.
@@ -46,8 +48,12 @@
 namespace bcr {
 namespace {
 
-typedef float f4_t __attribute__((ext_vector_type(4)));
-typedef float f16v __attribute__((ext_vector_type(16)));
+template <typename T>
+using V2 = T __attribute__((ext_vector_type(2)));
+template <typename T>
+using V4 = T __attribute__((ext_vector_type(4)));
+template <typename T>
+using V16 = T __attribute__((ext_vector_type(16)));
 
 // the most border rows (nr) the matrix-core pair takes
 constexpr int kNrB = 8;
@@ -64,26 +70,73 @@ struct MfmaGeom {
   static_assert(NT + NR <= 64 * RS, "padding slots come from the lanes beyond row R");
 };
 
-__device__ __forceinline__ f4_t mfma4(f4_t a, f4_t b, f4_t c) {
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b.z, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b.w, c, 0, 0, 0);
+// one 16 x 16 x 4 product-accumulate: FP32 or FP64 matrix cores (the same
+// operand and accumulator layouts)
+__device__ __forceinline__ V4<float> mfma1(float a, float b, V4<float> c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ V4<double> mfma1(double a, double b, V4<double> c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// The 16 x 16 x 4 products take A (16 x 4) as A[m][k] in lane m + 16 k and B
+// (4 x 16) as B[k][n] in lane n + 16 k, in both precisions; the 16 x 16
+// result comes back as four values per lane, lane n + 16 g holding in its
+// value i row 4 g + i (FP32) or row g + 4 i (FP64) of column n.  kro gives
+// that row; every k sum below runs in the same order (the i-th product of a
+// 16-wide block takes k = kro(g, i) in lane group g), so a 16-row block in
+// the result layout is the next product's B operand as it stands.
+template <typename T>
+__device__ __forceinline__ constexpr int kro(int g, int i) {
+  return sizeof(T) == 4 ? 4 * g + i : g + 4 * i;
+}
+// rows kro(g, 0..3) of a 16-row block (LDS): one 16-byte read for FP32,
+// four strided reads for FP64
+template <typename T>
+__device__ __forceinline__ V4<T> ld4(const T* blk, int g) {
+  if constexpr (sizeof(T) == 4) {
+    return *reinterpret_cast<const V4<T>*>(blk + 4 * g);
+  } else {
+    return V4<T>{blk[g], blk[g + 4], blk[g + 8], blk[g + 12]};
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ V4<T> mfma4(V4<T> a, V4<T> b, V4<T> c) {
+  c = mfma1(a.x, b.x, c);
+  c = mfma1(a.y, b.y, c);
+  c = mfma1(a.z, b.z, c);
+  c = mfma1(a.w, b.w, c);
   return c;
 }
 
-// rows 16 kb + 4 g + i (i < 4) of one column, those below lim: one 16-byte
-// store when the 16-row block lies below lim (a uniform test), single stores
-// in the block lim cuts
-__device__ __forceinline__ void store_block(float* col, int kb, int g, f4_t v, int lim) {
-  const int r0 = 16 * kb + 4 * g;
-  if (16 * kb + 16 <= lim) {
-    __builtin_memcpy(col + r0, &v, 16);   // 4-byte aligned; the hardware takes it
-  } else if (16 * kb < lim) {
+// rows 16 kb + kro(g, i) (i < 4) of one column, those below lim: for FP32 one
+// 16-byte store when the 16-row block lies below lim (a uniform test),
+// single stores in the block lim cuts
+template <typename T>
+__device__ __forceinline__ void store_block(T* col, int kb, int g, V4<T> v, int lim) {
+  if constexpr (sizeof(T) == 4) {
+    const int r0 = 16 * kb + 4 * g;
+    if (16 * kb + 16 <= lim) {
+      __builtin_memcpy(col + r0, &v, 4 * sizeof(T));   // element-aligned; the hardware takes it
+    } else if (16 * kb < lim) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (r0 + i < lim) col[r0 + i] = v[i];
+      for (int i = 0; i < 4; ++i)
+        if (r0 + i < lim) col[r0 + i] = v[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * kb + kro<T>(g, i);
+      if (r < lim) col[r] = v[i];
+    }
   }
+}
+
+// v when the mask is all ones, +0 when it is zero
+__device__ __forceinline__ float mask_bits(float v, int m) { return __int_as_float(__float_as_int(v) & m); }
+__device__ __forceinline__ double mask_bits(double v, int m) {
+  return __longlong_as_double(__double_as_longlong(v) & static_cast<long long>(m));
 }
 
 // lane `who`'s value of v when K > cc, else 0: a scalar select on the
@@ -95,19 +148,28 @@ __device__ __forceinline__ float rl_after(float v, int who, int cc) {
   asm("s_cmp_lt_u32 %1, %2\n\ts_cselect_b32 %0, %0, 0" : "+s"(x) : "s"(cc), "n"(K) : "scc");
   return __int_as_float(x);
 }
+template <int K>
+__device__ __forceinline__ double rl_after(double v, int who, int cc) {
+  const long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane(int(b & 0xffffffff), who);
+  int hi = __builtin_amdgcn_readlane(int(b >> 32), who);
+  asm("s_cmp_lt_u32 %2, %3\n\ts_cselect_b32 %0, %0, 0\n\ts_cselect_b32 %1, %1, 0"
+      : "+s"(lo), "+s"(hi) : "s"(cc), "n"(K) : "scc");
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
 
 // columns K.. 15 of one 16-column block of the row sets, those not after
 // column cc of the block getting a zero pivot-row entry; the pivot row is in
 // row set 0 or 1 (a third row set holds border rows only)
-template <int K, int RS>
-__device__ __forceinline__ void update_after(f16v& v0, f16v& v1, f16v& v2, bool sp, int who, int cc, float lv0,
-                                             float lv1, float lv2) {
+template <int K, int RS, typename TW>
+__device__ __forceinline__ void update_after(V16<TW>& v0, V16<TW>& v1, V16<TW>& v2, bool sp, int who, int cc, TW lv0,
+                                             TW lv1, TW lv2) {
   if constexpr (K < 16) {
-    const float u = rl_after<K>(sp ? v1[K] : v0[K], who, cc);
+    const TW u = rl_after<K>(sp ? v1[K] : v0[K], who, cc);
     v0[K] = fma(-lv0, u, v0[K]);
     v1[K] = fma(-lv1, u, v1[K]);
     if constexpr (RS > 2) v2[K] = fma(-lv2, u, v2[K]);
-    update_after<K + 1, RS>(v0, v1, v2, sp, who, cc, lv0, lv1, lv2);
+    update_after<K + 1, RS, TW>(v0, v1, v2, sp, who, cc, lv0, lv1, lv2);
   }
 }
 
@@ -115,25 +177,24 @@ __device__ __forceinline__ void update_after(f16v& v0, f16v& v1, f16v& v2, bool 
 // the rest rows (the minus sign is in Ls); the tiles' independent
 // accumulators alternate on the matrix core.  Reduced rows go to drest,
 // border rows back to their staging slots.
-template <int NB, int NRT, int NT, int LD, int XLD>
-__device__ __forceinline__ void rest_tiles(const f4_t (&u)[NB], const float* Ls, float* xt, int cl, int g, int n,
-                                           int nr, bool colok, float* drest) {
-  f4_t acc[NRT];
+template <int NB, int NRT, int NT, int LD, int XLD, typename TW>
+__device__ __forceinline__ void rest_tiles(const V4<TW> (&u)[NB], const TW* Ls, TW* xt, int cl, int g, int n, int nr,
+                                           bool colok, TW* drest) {
+  V4<TW> acc[NRT];
 #pragma unroll
-  for (int rt = 0; rt < NRT; ++rt) acc[rt] = *reinterpret_cast<const f4_t*>(xt + cl * XLD + NT + rt * 16 + 4 * g);
+  for (int rt = 0; rt < NRT; ++rt) acc[rt] = ld4(xt + cl * XLD + NT + rt * 16, g);
 #pragma unroll
   for (int kb = 0; kb < NB; ++kb) {
-    f4_t av[NRT];
+    V4<TW> av[NRT];
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
-      av[rt] = *reinterpret_cast<const f4_t*>(Ls + (NT + rt * 16 + cl) * LD + kb * 16 + 4 * g);
+      av[rt] = ld4(Ls + (NT + rt * 16 + cl) * LD + kb * 16, g);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int rt = 0; rt < NRT; ++rt)
-        acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rt][i], u[kb][i], acc[rt], 0, 0, 0);
+      for (int rt = 0; rt < NRT; ++rt) acc[rt] = mfma1(av[rt][i], u[kb][i], acc[rt]);
   }
-  // rest position q = 16 rt + 4 g + i: reduced row q (< n), else border row q - n
+  // rest position q = 16 rt + kro(g, i): reduced row q (< n), else border row q - n
   if (colok) {
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) store_block(drest, rt, g, acc[rt], n);
@@ -143,7 +204,7 @@ __device__ __forceinline__ void rest_tiles(const f4_t (&u)[NB], const float* Ls,
     if (16 * rt + 16 > n && 16 * rt < n + nr) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int q = rt * 16 + 4 * g + i;
+        const int q = rt * 16 + kro<TW>(g, i);
         if (q >= n && q < n + nr) xt[cl * XLD + NT + q] = acc[rt][i];
       }
     }
@@ -179,9 +240,8 @@ __device__ __forceinline__ void load_rhs_row(TS (&xs)[16], const TS* sD, const T
 }
 
 // ---- A: the T panel, one wave per pair ------------------------------------------
-template <typename TS, int NB, int RS, int NRMAX>
-__device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, int p) {
-  using TW = float;
+template <typename TS, typename TW, int NB, int RS, int NRMAX>
+__device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int p) {
   constexpr int NMAX = 16 * NB;
   static_assert(RS == 2 || RS == 3, "two or three row sets");
   static_assert(NMAX <= 64, "the 2n candidate rows lie in row sets 0 and 1");
@@ -205,7 +265,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
     if (!a.first) x += pend_prv[j * rn + e] + pend_cur[j * rn + e];
     stage[ri * NMAX + col] = x;
   }
-  f16v V[RS][NB];
+  V16<TW> V[RS][NB];
 #pragma unroll
   for (int s = 0; s < RS; ++s) {
     const int r = lane + 64 * s;
@@ -225,7 +285,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int k = 0; k < 16; k += 4) {
-          const f4_t c4 = *reinterpret_cast<const f4_t*>(srow + 16 * b + k);
+          const V4<TW> c4 = *reinterpret_cast<const V4<TW>*>(srow + 16 * b + k);
           V[s][b][k] += c4.x;
           V[s][b][k + 1] += c4.y;
           V[s][b][k + 2] += c4.z;
@@ -256,7 +316,8 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
       // the better of this lane's two candidates (ties to the smaller position)
       const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
       bool zero;
-      const int who = wave::argmax_pos<false>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane, zero);
+      const int who = wave::argmax_pos<(sizeof(TW) > 4)>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane,
+                                                          zero);
       const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
       const bool sp = __builtin_amdgcn_readlane(int(b1), who) != 0;
       zero_pivot = zero_pivot || zero;
@@ -281,7 +342,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
         V[RS - 1][b][cc] = u2 ? m2 : x2;
       }
       // columns 1..15 of the block (column 0 is never after c)
-      update_after<1, RS>(V[0][b], V[1][b], V[RS - 1][b], sp, who, cc, lv0, lv1, lv2);
+      update_after<1, RS, TW>(V[0][b], V[1][b], V[RS - 1][b], sp, who, cc, lv0, lv1, lv2);
 #pragma unroll
       for (int bb = b + 1; bb < NB; ++bb)
 #pragma unroll
@@ -314,9 +375,8 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, float> const& a, i
 }
 
 // ---- B: the right-hand part on the matrix cores ----------------------------------
-template <typename TS, int NB, int RS, int RESTMAX, int NW>
-__device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, int p) {
-  using TW = float;
+template <typename TS, typename TW, int NB, int RS, int RESTMAX, int NW>
+__device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p) {
   using G = MfmaGeom<16 * NB, RESTMAX, RS>;
   constexpr int NT = G::NT, NR = G::NR, LD = G::LD, XLD = G::XLD;
   __shared__ __align__(16) TW Ls[(NT + NR) * LD];
@@ -371,7 +431,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   {
     const int k = lane, kk = k < n ? k : n - 1;
     constexpr int PAIRS = (2 * NT + 2 * NW - 1) / (2 * NW);   // row pairs of T per wave
-    float2 tv[PAIRS];
+    V2<TW> tv[PAIRS];
 #pragma unroll
     for (int h = 0; h < PAIRS; ++h) {
       const int q0 = 2 * (w + NW * h), qc = q0 < n2 ? q0 : 0;
@@ -440,7 +500,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
       TW t0 = TW(0), t1 = TW(0);
 #pragma unroll
       for (int k0 = 0; k0 < i; k0 += 4) {
-        const f4_t l4 = *reinterpret_cast<const f4_t*>(li + k0);
+        const V4<TW> l4 = *reinterpret_cast<const V4<TW>*>(li + k0);
         t0 = fma(l4.x, y[k0], t0);
         if (k0 + 1 < 16) t1 = fma(l4.y, y[k0 + 1 < 16 ? k0 + 1 : 0], t1);
         if (k0 + 2 < 16) t0 = fma(l4.z, y[k0 + 2 < 16 ? k0 + 2 : 0], t0);
@@ -481,11 +541,12 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
       const bool valid = reg == 0 ? (cls & 5) != 0 : reg == 1 ? cls == 2 : (cls & 3) != 0;
       // a whole tile (a uniform test): one row mask for all columns (not for
       // NB = 2, where the second copy of the loop would cost a wave per SIMD)
-      if (NB > 2 && c0 + 16 <= width) {
+      // (FP32 only: the integer mask is one 32-bit AND per value)
+      if (sizeof(TW) == 4 && NB > 2 && c0 + 16 <= width) {
         int m = valid ? -1 : 0;
         asm volatile("" : "+v"(m));
 #pragma unroll
-        for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = __int_as_float(__float_as_int(TW(xs[s][cc])) & m);
+        for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = mask_bits(TW(xs[s][cc]), m);
       } else {
 #pragma unroll
         for (int cc = 0; cc < 16; ++cc) xt[cc * XLD + slot[s]] = keep_if(TW(xs[s][cc]), valid && c0 + cc < width);
@@ -533,15 +594,14 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     // U12 = L11^-1 X_top by 16-row blocks: u_kb = inv(L_kk) (X_kb - sum_{m<kb}
     // L_km u_m).  A block of U12 in the accumulator layout (rows 4g + i,
     // column cl) is the B operand layout of the next products as it stands.
-    f4_t u[NB];
+    V4<TW> u[NB];
 #pragma unroll
     for (int kb = 0; kb < NB; ++kb) {
-      f4_t acc = *reinterpret_cast<const f4_t*>(xt + cl * XLD + kb * 16 + 4 * g);
+      V4<TW> acc = ld4(xt + cl * XLD + kb * 16, g);
 #pragma unroll
       for (int mb = 0; mb < kb; ++mb)   // -L_km is in Ls
-        acc = mfma4(*reinterpret_cast<const f4_t*>(Ls + (kb * 16 + cl) * LD + mb * 16 + 4 * g), u[mb], acc);
-      u[kb] = mfma4(*reinterpret_cast<const f4_t*>(Ls + (kb * 16 + cl) * LD + kb * 16 + 4 * g), acc,
-                    f4_t{0.f, 0.f, 0.f, 0.f});
+        acc = mfma4<TW>(ld4(Ls + (kb * 16 + cl) * LD + mb * 16, g), u[mb], acc);
+      u[kb] = mfma4<TW>(ld4(Ls + (kb * 16 + cl) * LD + kb * 16, g), acc, V4<TW>{TW(0), TW(0), TW(0), TW(0)});
     }
     if (colok) {
 #pragma unroll
@@ -551,9 +611,9 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
     // the 16-row tiles the n + nr rest rows occupy (one fewer than NR / 16
     // when n + nr <= NR - 16)
     if (n + nr <= NR - 16)
-      rest_tiles<NB, NR / 16 - 1, NT, LD, XLD>(u, Ls, xt, cl, g, n, nr, colok, drest);
+      rest_tiles<NB, NR / 16 - 1, NT, LD, XLD, TW>(u, Ls, xt, cl, g, n, nr, colok, drest);
     else
-      rest_tiles<NB, NR / 16, NT, LD, XLD>(u, Ls, xt, cl, g, n, nr, colok, drest);
+      rest_tiles<NB, NR / 16, NT, LD, XLD, TW>(u, Ls, xt, cl, g, n, nr, colok, drest);
     // border rows: C_jp' (region 0), pending C_jpp += (region 1), F partial (region 2)
     TW* const bdst = reg == 0 ? Cm + jp * rn + eb : reg == 1 ? pend_cur + jpp * rn + eb
                                                               : Fm + size_t(j) * nr * nx + eb;
@@ -569,23 +629,23 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, float> const& a, in
   }
 }
 
-template <typename TS, int NB, int RS, int NRMAX>
-__global__ void __launch_bounds__(64) pair_lu_t_kernel(FactorArgs<TS, float> a) {
-  pair_lu_t_body<TS, NB, RS, NRMAX>(a, blockIdx.x);
+template <typename TS, typename TW, int NB, int RS, int NRMAX>
+__global__ void __launch_bounds__(64) pair_lu_t_kernel(FactorArgs<TS, TW> a) {
+  pair_lu_t_body<TS, TW, NB, RS, NRMAX>(a, blockIdx.x);
 }
-template <typename TS, int NB, int RS, int NRMAX>
-__global__ void __launch_bounds__(64) pair_lu_t_batched(BatchTable<FactorArgs<TS, float>> t) {
+template <typename TS, typename TW, int NB, int RS, int NRMAX>
+__global__ void __launch_bounds__(64) pair_lu_t_batched(BatchTable<FactorArgs<TS, TW>> t) {
   const int e = t.entry[blockIdx.x];
-  pair_lu_t_body<TS, NB, RS, NRMAX>(t.args[e], int(blockIdx.x) - t.first[e]);
+  pair_lu_t_body<TS, TW, NB, RS, NRMAX>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
-template <typename TS, int NB, int RS, int RESTMAX, int NW>
-__global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_kernel(FactorArgs<TS, float> a) {
-  pair_rhs_body<TS, NB, RS, RESTMAX, NW>(a, blockIdx.x);
+template <typename TS, typename TW, int NB, int RS, int RESTMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_kernel(FactorArgs<TS, TW> a) {
+  pair_rhs_body<TS, TW, NB, RS, RESTMAX, NW>(a, blockIdx.x);
 }
-template <typename TS, int NB, int RS, int RESTMAX, int NW>
-__global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_batched(BatchTable<FactorArgs<TS, float>> t) {
+template <typename TS, typename TW, int NB, int RS, int RESTMAX, int NW>
+__global__ void __launch_bounds__(64 * NW) pair_rhs_mfma_batched(BatchTable<FactorArgs<TS, TW>> t) {
   const int e = t.entry[blockIdx.x];
-  pair_rhs_body<TS, NB, RS, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
+  pair_rhs_body<TS, TW, NB, RS, RESTMAX, NW>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
 // waves per pair in the right-hand kernel
@@ -604,25 +664,39 @@ int mfma_variant(int n, int nr, int nx) {
 
 }  // namespace
 
-bool pair_lu_mfma_fits(int n, int nr, int nx) { return mfma_variant(n, nr, nx) != 0; }
+bool pair_lu_mfma_fits(int n, int nr, int nx, bool fp64) {
+  const int v = mfma_variant(n, nr, nx);
+  return fp64 ? v == 1 || v == 2 : v != 0;
+}
 
-template <typename TS>
-bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err) {
+namespace {
+// the A and B launches of shape variant V (FP64 factors: V = 1, 2; the T
+// panel of V = 3 would need 384 registers per lane in FP64)
+template <int V, typename TS, typename TW>
+void launch_pair(FactorArgs<TS, TW> const& a, hipStream_t st) {
+  constexpr int NB = V + 1, RS = V == 3 ? 3 : 2, RESTMAX = 32 + 16 * V;
+  hipLaunchKernelGGL((pair_lu_t_kernel<TS, TW, NB, RS, kNrB>), dim3(a.npairs), dim3(64), 0, st, a);
+  hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, TW, NB, RS, RESTMAX, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves),
+                     0, st, a);
+}
+template <int V, typename TS, typename TW>
+void launch_pair_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, hipStream_t st) {
+  constexpr int NB = V + 1, RS = V == 3 ? 3 : 2, RESTMAX = 32 + 16 * V;
+  hipLaunchKernelGGL((pair_lu_t_batched<TS, TW, NB, RS, kNrB>), dim3(nblocks), dim3(64), 0, st, t);
+  hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, TW, NB, RS, RESTMAX, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves),
+                     0, st, t);
+}
+}  // namespace
+
+template <typename TS, typename TW>
+bool launch_pair_lu_mfma(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err) {
+  constexpr bool f64 = sizeof(TW) == 8;
   switch (mfma_variant(a.n, a.nr, a.nx)) {
-    case 1:
-      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 2, 2, 8>), dim3(a.npairs), dim3(64), 0, st, a);
-      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 2, 2, 48, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
-                         a);
-      break;
-    case 2:
-      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 3, 2, 8>), dim3(a.npairs), dim3(64), 0, st, a);
-      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 3, 2, 64, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
-                         a);
-      break;
+    case 1: launch_pair<1>(a, st); break;
+    case 2: launch_pair<2>(a, st); break;
     case 3:
-      hipLaunchKernelGGL((pair_lu_t_kernel<TS, 4, 3, 8>), dim3(a.npairs), dim3(64), 0, st, a);
-      hipLaunchKernelGGL((pair_rhs_mfma_kernel<TS, 4, 3, 80, kRhsWaves>), dim3(a.npairs), dim3(64 * kRhsWaves), 0, st,
-                         a);
+      if constexpr (f64) return false;
+      else launch_pair<3>(a, st);
       break;
     default:
       return false;
@@ -631,24 +705,16 @@ bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipErro
   return true;
 }
 
-template <typename TS>
-bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int nblocks, int n, int nx, int nr,
+template <typename TS, typename TW>
+bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, TW>> const& t, int nblocks, int n, int nx, int nr,
                                  hipStream_t st, hipError_t* err) {
+  constexpr bool f64 = sizeof(TW) == 8;
   switch (mfma_variant(n, nr, nx)) {
-    case 1:
-      hipLaunchKernelGGL((pair_lu_t_batched<TS, 2, 2, 8>), dim3(nblocks), dim3(64), 0, st, t);
-      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 2, 2, 48, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
-                         t);
-      break;
-    case 2:
-      hipLaunchKernelGGL((pair_lu_t_batched<TS, 3, 2, 8>), dim3(nblocks), dim3(64), 0, st, t);
-      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 3, 2, 64, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
-                         t);
-      break;
+    case 1: launch_pair_batched<1>(t, nblocks, st); break;
+    case 2: launch_pair_batched<2>(t, nblocks, st); break;
     case 3:
-      hipLaunchKernelGGL((pair_lu_t_batched<TS, 4, 3, 8>), dim3(nblocks), dim3(64), 0, st, t);
-      hipLaunchKernelGGL((pair_rhs_mfma_batched<TS, 4, 3, 80, kRhsWaves>), dim3(nblocks), dim3(64 * kRhsWaves), 0, st,
-                         t);
+      if constexpr (f64) return false;
+      else launch_pair_batched<3>(t, nblocks, st);
       break;
     default:
       return false;
@@ -657,11 +723,14 @@ bool launch_pair_lu_mfma_batched(BatchTable<FactorArgs<TS, float>> const& t, int
   return true;
 }
 
-template bool launch_pair_lu_mfma<float>(FactorArgs<float, float> const&, hipStream_t, hipError_t*);
-template bool launch_pair_lu_mfma<double>(FactorArgs<double, float> const&, hipStream_t, hipError_t*);
-template bool launch_pair_lu_mfma_batched<float>(BatchTable<FactorArgs<float, float>> const&, int, int, int, int,
-                                                 hipStream_t, hipError_t*);
-template bool launch_pair_lu_mfma_batched<double>(BatchTable<FactorArgs<double, float>> const&, int, int, int,
-                                                  int, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma<float, float>(FactorArgs<float, float> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma<double, float>(FactorArgs<double, float> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma<double, double>(FactorArgs<double, double> const&, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma_batched<float, float>(BatchTable<FactorArgs<float, float>> const&, int, int, int, int,
+                                                        hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma_batched<double, float>(BatchTable<FactorArgs<double, float>> const&, int, int, int,
+                                                         int, hipStream_t, hipError_t*);
+template bool launch_pair_lu_mfma_batched<double, double>(BatchTable<FactorArgs<double, double>> const&, int, int,
+                                                          int, int, hipStream_t, hipError_t*);
 
 }  // namespace bcr

@@ -9,6 +9,12 @@ template <typename TW, typename TX> struct SolveArgs;
 template <typename TW, typename TA> struct AssembleArgs;
 template <typename TA> struct MvArgs;
 
+// Householder QR / QRP pair elimination for every n <= 64, nr <= 16: T in
+// LDS, the right-hand columns in chunks of cw next to it (pair_qr_plan picks
+// cw and the LDS bytes); false for LU or a shape beyond
+bool pair_qr_plan(int n, int nr, int nx, size_t elem, size_t lds_limit, int* cw, size_t* bytes);
+template <typename TS, typename TW>
+bool launch_pair_qr(int method, FactorArgs<TS, TW> const& a, size_t lds_limit, hipStream_t st, hipError_t* err);
 template <typename TS, typename TW>
 hipError_t launch_pair_generic(int method, FactorArgs<TS, TW> const& a, TW* gscratch, size_t lds_limit,
                                hipStream_t st);
@@ -25,11 +31,13 @@ bool launch_pair_lu_tiled(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_
 bool pair_lu_cols_fits(int n, int nr, int nx);
 template <typename TS, typename TW>
 bool launch_pair_lu_cols(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
-// FP32 factors, n <= 48, n + nr <= 64, nx <= 16: one wave factors T, the
-// right-hand columns are two GEMMs on the matrix cores; false otherwise
-bool pair_lu_mfma_fits(int n, int nr, int nx);
-template <typename TS>
-bool launch_pair_lu_mfma(FactorArgs<TS, float> const& a, hipStream_t st, hipError_t* err);
+// the matrix-core pair: one wave factors T, the right-hand columns are
+// blocked triangular solves and GEMMs on the matrix cores.  FP32 factors:
+// n <= 64, n + nr <= 80, nr <= 8, nx <= 16; FP64 factors: n <= 48,
+// n + nr <= 64; false otherwise
+bool pair_lu_mfma_fits(int n, int nr, int nx, bool fp64);
+template <typename TS, typename TW>
+bool launch_pair_lu_mfma(FactorArgs<TS, TW> const& a, hipStream_t st, hipError_t* err);
 
 template <typename TW, typename TX>
 hipError_t launch_forward(int method, SolveArgs<TW, TX> const& a, hipStream_t st);

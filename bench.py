@@ -550,7 +550,9 @@ def sub_single_fp64(args):
                 rel_residual=res, max_fwd_err_vs_ones=float(np.abs(x - 1).max()),
                 pair_kernel_gbs=byts / launches / (avg * 1e-3) / 1e9,
                 pair_kernel_frac_hbm=byts / launches / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                pair_ms_avg_launch=avg)
+                pair_ms_avg_launch=avg,
+                pair_traffic_bytes_per_launch=profiled_traffic("pair_mfma_fp64_N100000_n32.md"),
+                pair_flop_rate_tflops=(N - 1) * pair_flops(n, r, r) / launches / (avg * 1e-3) / 1e12)
 
 
 def sub_mixed_n64(args):
@@ -649,6 +651,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-sample-N", type=int, default=25000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--method", choices=["LU", "QR", "QRP"], default="LU",
+                    help="single workload: the pair elimination method")
     ap.add_argument("--mode", choices=["fp64", "mixed"], default="fp64",
                     help="fp64: plain FP64 cyclic reduction; mixed: FP32 factors + FP64 refinement to 1e-12")
     ap.add_argument("--kappa", type=float, default=0.0,
@@ -702,7 +706,7 @@ def main():
 
     g = BorderedCR()
     g.allocate(N, n, qr, qx, nr, nx)
-    g.select("LU", "LU", "NONE")
+    g.select(args.method, "LU", "NONE")
     if args.mode == "mixed":
         g.select_precision("MIXED", 1e-12, 30)
     g.set_pair_kernel(args.pair_kernel)
@@ -812,7 +816,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            # counter bytes per launch from the committed rocprof summary of the
+            # default configuration (FP64, N = 100000, n = 32, default kernel)
+            "traffic": (profiled_traffic("pair_mfma_fp64_N100000_n32.md")
+                        if (args.mode == "fp64" and args.method == "LU" and args.pair_kernel == 0
+                            and (N, n, qr, nr) == (100000, 32, 8, 4)) else None),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": pair_ms_avg_launch,
             "launches_per_step": launches,

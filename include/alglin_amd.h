@@ -136,14 +136,13 @@ int alglin_bcr_get_factors(alglin_bcr_t h, void* T, int32_t* perm, void* D, void
                            double* last_block, double* pending_sumsq);
 /* 1 when the last factorization used the register-resident LU kernel */
 int alglin_bcr_fast_path(alglin_bcr_t h);
-/* LU pair kernel: 0 (default) the matrix-core pair for FP32 factors with
-   16 < n <= 64 where it applies (nr <= 8, nx <= 16), else register kernels
-   for n <= 32, the column-split-waves kernel up to n = 64 and the
-   register-tiled kernel beyond; 3 the register-tiled kernel for every n; 4 the
-   column-split-waves kernel for every n <= 64; 5 the matrix-core pair for
-   FP32 factors with n <= 64, nr <= 8, nx <= 16, n + nr <= 48 / 64 / 80 for
-   n <= 32 / 48 / 64 (FP64 factors and other shapes: the register-tiled
-   kernel) */
+/* LU pair kernel: 0 (default) the matrix-core pair for 16 < n where it
+   applies (FP32 factors n <= 64, FP64 factors n <= 48; nr <= 8, nx <= 16),
+   else register kernels for n <= 32, the column-split-waves kernel up to
+   n = 64 and the register-tiled kernel beyond; 3 the register-tiled kernel
+   for every n; 4 the column-split-waves kernel for every n <= 64; 5 the
+   matrix-core pair where it applies (n + nr <= 48 / 64 / 80 for n <= 32 / 48
+   / 64; other shapes: the register-tiled kernel) */
 int alglin_bcr_set_pair_kernel(alglin_bcr_t h, int variant);
 
 /* ---- precision (extension of the reference surface) -------------------------- */
@@ -264,8 +263,9 @@ int alglin_batch_time_factorize(alglin_batch_t h, double* rows, int32_t max_rows
 /* Kernels of the 16 < n <= 32 and 32 < n <= 64 LU pair classes: 0 default
    (= 5 where it applies, else 4), 4 the rank-1 kernels (two-wave split for
    n <= 32, column-split waves beyond: every column updated per step), 5 the
-   matrix-core pair (FP32 factors, n <= 48, nr <= 8, nx <= 16, n + nr <= 48
-   for n <= 32 and <= 64 beyond; other shapes take the rank-1 kernels).
+   matrix-core pair (FP32 or FP64 factors, n <= 48, nr <= 8, nx <= 16,
+   n + nr <= 48 for n <= 32 and <= 64 beyond; other shapes take the rank-1
+   kernels).
    Drops captured graphs.  No counterpart in the reference (a tuning hook). */
 int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant);
 

@@ -46,18 +46,28 @@ def oracle_x(oracle_mod, sysd, method, b):
     return x
 
 
-def test_batch_fp64_matches_oracle_per_system(oracle_mod):
+@pytest.mark.parametrize("kernel", ["tuned", "mfma"])
+def test_batch_fp64_matches_oracle_per_system(oracle_mod, kernel):
+    """FP64 batch against the oracle per system, with the tuned class kernels
+    and with the matrix-core pair forced on the 16 < n <= 32 and 32 < n <= 48
+    LU classes (FP64 MFMA for the right-hand blocks)."""
     dims = mixed_dims()
-    print(f"case batch fp64 nsys={len(dims)}", flush=True)
+    print(f"case batch fp64 nsys={len(dims)} kernel={kernel}", flush=True)
     bt = Batch(mixed=False)
     bt.plan(dims)
+    if kernel == "mfma":
+        bt.set_class_kernel(3, 0, fp32_factors=False)
+        bt.set_class_kernel(4, 0, fp32_factors=False)
     bt.generate(seed=7, shift=4.0)
     b = random_rhs(bt, 1)
     B = DeviceArray.from_host(b)
     X = DeviceArray(bt.size("x"))
-    rc = bt.solve(B, X, tol=1e-12, max_iter=3, graph=False)
+    # no refinement sweep: the first FP64 solve itself must match the oracle
+    # (refinement would hide wrong factors that still precondition well)
+    rc = bt.solve(B, X, tol=1e-12, max_iter=0, graph=False)
     rep = bt.report()
     assert rc == 0, rep
+    assert rep["sweeps"].max() == 0
     x = X.to_host()
     for s, row in enumerate(dims):
         sysd = bt.system(s)
