@@ -322,9 +322,9 @@ class BorderedCR:
 
     def set_pair_kernel(self, variant: int):
         """LU pair kernel: 0 default, 3 register tiles for every n,
-        4 column-split waves for every n <= 64, 5 the matrix-core pair (FP32 factors,
-        n <= 64, nr <= 8, nx <= 16).  The default takes the matrix-core pair for FP32
-        factors with 16 < n <= 64 where it applies."""
+        4 column-split waves for every n <= 64, 5 the matrix-core pair (FP32 factors
+        n <= 64, FP64 factors n <= 48; nr <= 8, nx <= 16).  The default takes the
+        matrix-core pair for 16 < n where it applies."""
         self._check(lib().alglin_bcr_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
     def set_fallback(self, on: bool):

@@ -60,6 +60,9 @@ struct SolveArgs {
   TX* Xpart;          // [N][nr*nrhs]
   const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
   const int32_t* pairs = nullptr;  // as FactorArgs::pairs
+  // back substitution of QRP pairs (streamed LU sweep): after U^-1, the
+  // pair's column exchanges (perm: 0-based exchange list) act on x_j
+  int exch = 0;
 };
 
 // Arguments of the last-block assembly.

@@ -554,10 +554,16 @@ void Engine<TA, TW>::solve_async(TX* x, int nrhs, int64_t ldx) {
     a.pairs = it->off >= 0 ? pairtab_.p + it->off : nullptr;
     timed(kBackward, [&] {
       hipError_t e = hipSuccess;
-      if (method_ == kLU && sweep_fast_ &&
+      // the back substitution of a QR pair is the LU one (R where U sits);
+      // QRP adds the pair's column exchanges (the streamed sweep's exch)
+      a.exch = method_ == kQRP ? 1 : 0;
+      if ((method_ == kLU || method_ == kQR) && sweep_fast_ &&
           (launch_sweep_lu_fast(false, a, st_, &e) || launch_sweep_lu_stream(false, a, st_, &e)))
         hip_check(e, "backward_fast");
+      else if (method_ == kQRP && sweep_fast_ && launch_sweep_lu_stream(false, a, st_, &e))
+        hip_check(e, "backward_stream");
       else hip_check(launch_backward(method_, a, st_), "backward");
+      a.exch = 0;
     });
   }
 }

@@ -161,17 +161,28 @@ __device__ __forceinline__ void mv_tail_body(MvArgs<TA> const& a, const double* 
   const double* xN = a.x + size_t(N) * n;
   const double* xq = a.x + size_t(N + 1) * n;
   const double* xb = xq + qx;
+  // x entry of boundary column c (x_0, x_N, q, xb in turn; c < Nc)
+  auto xcol = [&](int c) -> double {
+    return c < n ? x0[c] : c < 2 * n ? xN[c - n] : c < 2 * n + qx ? xq[c - 2 * n] : xb[c - 2 * n - qx];
+  };
   for (int i = threadIdx.x; i < m + nr; i += blockDim.x) {
     double acc = 0;
     size_t row;
     if (i < m) {
-      for (int c = 0; c < Nc; ++c) {
-        double xv;
-        if (c < n) xv = x0[c];
-        else if (c < 2 * n) xv = xN[c - n];
-        else if (c < 2 * n + qx) xv = xq[c - 2 * n];
-        else xv = xb[c - 2 * n - qx];
-        acc += mag<ABS>(double(a.H[i + size_t(c) * m])) * mag<ABS>(xv);
+      // eight columns' loads in flight at a time (clamped columns, the
+      // products of the extra ones dropped); the sum keeps column order
+      constexpr int U = 8;
+      for (int c0 = 0; c0 < Nc; c0 += U) {
+        double hv[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int c = c0 + u < Nc ? c0 + u : Nc - 1;
+          hv[u] = double(a.H[i + size_t(c) * m]);
+          xv[u] = xcol(c);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (c0 + u < Nc) acc += mag<ABS>(hv[u]) * mag<ABS>(xv[u]);
       }
       row = size_t(N) * n + i;
     } else {

@@ -27,7 +27,6 @@ __device__ __forceinline__ void forward_body(SolveArgs<TW, TX> a, int bid) {
   const int tid = threadIdx.x, nth = blockDim.x;
   TW* Tp = reinterpret_cast<TW*>(smem_raw);                                     // 2n*n + n
   TX* X = reinterpret_cast<TX*>(smem_raw + align16(tsize(n) * sizeof(TW)));   // 2n*kRhsChunk
-  __shared__ TX s_w[kRhsChunk];
   __shared__ int s_perm[64];
 
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
@@ -59,22 +58,31 @@ __device__ __forceinline__ void forward_body(SolveArgs<TW, TX> a, int bid) {
         __syncthreads();
       }
     } else {
-      // Q^T = H_{n-1} ... H_0 (ormqr 'L','T', cc:962-971)
-      for (int c = 0; c < n; ++c) {
-        const TX tau = TX(Tp[size_t(n2) * n + c]);
-        for (int q = tid; q < nq; q += nth) {
-          TX w = X[c + q * n2];
-          for (int r = c + 1; r < n2; ++r) w += TX(Tp[r + c * n2]) * X[r + q * n2];
-          s_w[q] = w * tau;
+      // Q^T = H_{n-1} ... H_0 (ormqr 'L','T', cc:962-971): one wave per
+      // right-hand side, its rows lane and lane + 64 (2n <= 128) in
+      // registers; per reflector a wave dot product (a butterfly: every
+      // lane ends with the same sum) and an axpy, no barrier
+      const int lane = tid & 63, wv = tid >> 6, nw = nth >> 6;
+      for (int q = wv; q < nq; q += nw) {
+        TX* xq = X + q * n2;
+        TX x0 = lane < n2 ? xq[lane] : TX(0);
+        TX x1 = lane + 64 < n2 ? xq[lane + 64] : TX(0);
+        for (int c = 0; c < n; ++c) {
+          const TX tau = TX(Tp[size_t(n2) * n + c]);
+          // v = (1 at row c, the stored reflector below it); row c < 64
+          const TX v0 = lane == c ? TX(1) : (lane > c && lane < n2 ? TX(Tp[lane + c * n2]) : TX(0));
+          const TX v1 = lane + 64 < n2 ? TX(Tp[lane + 64 + c * n2]) : TX(0);
+          TX d = fma(v0, x0, v1 * x1);
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) d += __shfl_xor(d, off);
+          const TX w = d * tau;
+          x0 = fma(-v0, w, x0);
+          x1 = fma(-v1, w, x1);
         }
-        __syncthreads();
-        for (int i = tid; i < (n2 - c) * nq; i += nth) {
-          const int r = c + i % (n2 - c), q = i / (n2 - c);
-          const TX v = r == c ? TX(1) : TX(Tp[r + c * n2]);
-          X[r + q * n2] -= v * s_w[q];
-        }
-        __syncthreads();
+        if (lane < n2) xq[lane] = x0;
+        if (lane + 64 < n2) xq[lane + 64] = x1;
       }
+      __syncthreads();
     }
     // TOP (x_jp) <- W[n:2n], BOTTOM (x_j) <- W[0:n]   (cc:1041-1042)
     for (int i = tid; i < n2 * nq; i += nth) {

@@ -112,6 +112,19 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
   // the reciprocal of this lane's diagonal of U, once per pair: the bottom-up
   // recurrence multiplies instead of dividing (x_c = s_c * (1/u_cc))
   const TX dinv = row ? TX(1) / TX(Tj[rc + size_t(rc) * n2]) : TX(1);
+  // QRP: y_i <-> y_{s_i} for i = 0..n-1 with s_i > i (cc:1851-1856), composed
+  // into the lane whose value ends in row `lane`
+  int src = lane;
+  if (a.exch) {
+    const int sl = row ? a.perm[size_t(j) * n + lane] : 0;
+    for (int i = 0; i < n; ++i) {
+      const int s = __builtin_amdgcn_readlane(sl, i);
+      if (s > i && s < n) {
+        const int si = __builtin_amdgcn_readlane(src, i), ss = __builtin_amdgcn_readlane(src, s);
+        src = lane == i ? ss : lane == s ? si : src;
+      }
+    }
+  }
   for (int q = 0; q < nrhs; ++q) {
     TX* x = a.x + size_t(q) * a.ldx;
     const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
@@ -165,6 +178,7 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
         }
       }
     }
+    if (a.exch) s = __shfl(s, src);
     if (row) x[size_t(j) * n + lane] = s;
   }
 }

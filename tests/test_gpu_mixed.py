@@ -118,9 +118,9 @@ def test_mixed_reports_stall_on_fp32_singular_blocks():
 @pytest.mark.parametrize("N,n,q,r,kmax", [(64, 8, 2, 2, 1e4), (20, 64, 16, 8, 1e6), (33, 40, 8, 4, 1e5)])
 def test_mixed_falls_back_to_fp64_qrp(oracle_mod, N, n, q, r, kmax):
     """max_iter = 0 ends the refinement after the first FP32 solve, so every
-    column takes the FP64 QRP path (n = 64: the panel in the generic kernel's
-    global scratch): the result is that path's, status 4, agreeing with the
-    FP64 QRP oracle and at its residual."""
+    column takes the FP64 QRP path (the chunked QR pair kernel, T and a
+    column chunk in LDS, also at n = 64): the result is that path's, status
+    4, agreeing with the FP64 QRP oracle and at its residual."""
     print(f"case N={N} n={n} q={q} r={r} kmax={kmax:g}", flush=True)
     s = systems.conditioned_system(N, n, q, q, r, r, kmin=1e2, kmax=kmax, seed=3 + n)
     nrows, _ = systems.dims(s)
@@ -142,9 +142,15 @@ def test_mixed_falls_back_to_fp64_qrp(oracle_mod, N, n, q, r, kmax):
     res = [np.linalg.norm(systems.mv(s, X[:, c]) - b[:, c]) / np.linalg.norm(b[:, c]) for c in range(2)]
     assert max(res) <= tol, (res, floor)
     # a residual at this level is mostly the rounding of its own evaluation:
-    # the device's and numpy's summation orders agree to a few percent
-    assert abs(rep["rel_residual"] - max(res)) <= 0.1 * max(res) + 1e-16
+    # two FP64 evaluations of b - A x (the device's, numpy's) differ by up to
+    # a few u |A| |x| + |b| in norm, relative to ||b||
     A = systems.dense(s) if nrows <= 3000 else None
+    if A is not None:
+        noise = max(8 * np.finfo(float).eps * np.linalg.norm(np.abs(A) @ np.abs(X[:A.shape[1], c]) + np.abs(b[:, c]))
+                    / np.linalg.norm(b[:, c]) for c in range(2))
+    else:
+        noise = 1e-13
+    assert abs(rep["rel_residual"] - max(res)) <= 0.1 * max(res) + noise, (rep["rel_residual"], res, noise)
     cond = np.linalg.cond(A) if A is not None else 1e8
     assert np.abs(X - Xo).max() / np.abs(Xo).max() <= max(1e-12, 4 * cond * tol)
 
