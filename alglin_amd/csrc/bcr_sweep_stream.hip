@@ -33,7 +33,10 @@ __device__ __forceinline__ double wsum(double v) {
 // columns loaded per chunk: one memory latency per chunk
 constexpr int kChunk = 16;
 
-template <typename TW, typename TX>
+// QB right-hand sides per pass: every factor chunk is loaded once for all
+// of them (QB = 1 for one right-hand side; the last pass of several masks
+// the columns past nrhs)
+template <typename TW, typename TX, int QB>
 __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int bid) {
   const int lane = threadIdx.x & 63;
   const int p = bid * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));   // wave-uniform
@@ -62,10 +65,14 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
   // factor columns kChunk at a time, from clamped addresses: one memory
   // latency per chunk (rows past 2n hold garbage that is never used)
   const int r0c = v0 ? r0 : 0, r1c = v1 ? r1 : 0;
-  for (int q = 0; q < nrhs; ++q) {
-    TX* x = a.x + size_t(q) * a.ldx;
-    TX x0 = v0 ? x[in_row(src0)] : TX(0);
-    TX x1 = v1 ? x[in_row(src1)] : TX(0);
+  for (int q0 = 0; q0 < nrhs; q0 += QB) {
+    TX x0[QB], x1[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const TX* x = a.x + size_t(q0 + b < nrhs ? q0 + b : nrhs - 1) * a.ldx;
+      x0[b] = v0 ? x[in_row(src0)] : TX(0);
+      x1[b] = v1 ? x[in_row(src1)] : TX(0);
+    }
     for (int c0 = 0; c0 < n; c0 += kChunk) {
       TW t0[kChunk], t1[kChunk];
 #pragma unroll
@@ -78,22 +85,32 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
       for (int i = 0; i < kChunk; ++i) {
         const int c = c0 + i;
         if (c < n) {
-          const TX xc = TX(rlane(double(x0), c));
-          if (r0 > c) x0 -= TX(t0[i]) * xc;
-          x1 -= TX(t1[i]) * xc;
+#pragma unroll
+          for (int b = 0; b < QB; ++b) {
+            const TX xc = TX(rlane(double(x0[b]), c));
+            if (r0 > c) x0[b] -= TX(t0[i]) * xc;
+            x1[b] -= TX(t1[i]) * xc;
+          }
         }
       }
     }
-    if (v0) x[out_row(r0)] = x0;
-    if (v1) x[out_row(r1)] = x1;
-    for (int i = 0; i < nr; ++i) {
-      const double s = wsum(lane < n ? double(Cj[i + size_t(lane) * nr]) * double(x0) : 0.0);
-      if (lane == 0) a.Xpart[size_t(j) * nr * nrhs + i + size_t(q) * nr] = TX(-s);
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const int q = q0 + b;
+      if (q < nrhs) {
+        TX* x = a.x + size_t(q) * a.ldx;
+        if (v0) x[out_row(r0)] = x0[b];
+        if (v1) x[out_row(r1)] = x1[b];
+        for (int i = 0; i < nr; ++i) {
+          const double s = wsum(lane < n ? double(Cj[i + size_t(lane) * nr]) * double(x0[b]) : 0.0);
+          if (lane == 0) a.Xpart[size_t(j) * nr * nrhs + i + size_t(q) * nr] = TX(-s);
+        }
+      }
     }
   }
 }
 
-template <typename TW, typename TX>
+template <typename TW, typename TX, int QB>
 __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int bid) {
   const int lane = threadIdx.x & 63;
   const int p = bid * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));   // wave-uniform
@@ -125,76 +142,154 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
       }
     }
   }
-  for (int q = 0; q < nrhs; ++q) {
-    TX* x = a.x + size_t(q) * a.ldx;
-    const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
-    const TX vjpp = row ? x[size_t(jpp) * n + lane] : TX(0);
-    TX s = row ? x[size_t(j) * n + lane] : TX(0);
-    // D_j, E_j columns kChunk at a time (one memory latency per chunk)
-    for (int c0 = 0; c0 < n; c0 += kChunk) {
-      TW d[kChunk], e[kChunk];
-#pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
-        const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n;
-        d[i] = gload(Dj, rc + cc);
-        e[i] = gload(Ej, rc + cc);
-      }
-#pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
-        const int c = c0 + i;
-        if (c < n) s -= TX(d[i]) * TX(rlane(double(vjp), c)) + TX(e[i]) * TX(rlane(double(vjpp), c));
-      }
-    }
-    // B_j xb, eight columns' loads in flight at a time
-    for (int c0 = 0; c0 < nx; c0 += 8) {
-      TW bv[8];
-      TX xv[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int cc = c0 + i < nx ? c0 + i : 0;
-        bv[i] = gload(Bj, rc + unsigned(cc) * n);
-        xv[i] = x[xbrow + cc];
-      }
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (c0 + i < nx) s -= TX(keep_if(bv[i], row)) * xv[i];
-    }
-    // U^{-1}, bottom up: lane c owns the diagonal of column c; columns kChunk
-    // at a time from the top of the remaining range down
-    for (int c1 = n - 1; c1 >= 0; c1 -= kChunk) {
-      TW u[kChunk];
-#pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
-        const int c = c1 - i;
-        u[i] = gload(Tj, rc + unsigned(c >= 0 ? c : 0) * n2);
-      }
-#pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
-        const int c = c1 - i;
-        if (c >= 0) {
-          const TX xc = TX(rlane(double(s * dinv), c));
-          if (lane < c) s -= TX(u[i]) * xc;
-          else if (lane == c) s = xc;
+  if constexpr (QB == 1) {   // one column: the per-column loop as tuned
+    for (int q = 0; q < nrhs; ++q) {
+      TX* x = a.x + size_t(q) * a.ldx;
+      const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
+      const TX vjpp = row ? x[size_t(jpp) * n + lane] : TX(0);
+      TX s = row ? x[size_t(j) * n + lane] : TX(0);
+      // D_j, E_j columns kChunk at a time (one memory latency per chunk)
+      for (int c0 = 0; c0 < n; c0 += kChunk) {
+        TW d[kChunk], e[kChunk];
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n;
+          d[i] = gload(Dj, rc + cc);
+          e[i] = gload(Ej, rc + cc);
+        }
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const int c = c0 + i;
+          if (c < n) s -= TX(d[i]) * TX(rlane(double(vjp), c)) + TX(e[i]) * TX(rlane(double(vjpp), c));
         }
       }
+      // B_j xb, eight columns' loads in flight at a time
+      for (int c0 = 0; c0 < nx; c0 += 8) {
+        TW bv[8];
+        TX xv[8];
+  #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int cc = c0 + i < nx ? c0 + i : 0;
+          bv[i] = gload(Bj, rc + unsigned(cc) * n);
+          xv[i] = x[xbrow + cc];
+        }
+  #pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (c0 + i < nx) s -= TX(keep_if(bv[i], row)) * xv[i];
+      }
+      // U^{-1}, bottom up: lane c owns the diagonal of column c; columns kChunk
+      // at a time from the top of the remaining range down
+      for (int c1 = n - 1; c1 >= 0; c1 -= kChunk) {
+        TW u[kChunk];
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const int c = c1 - i;
+          u[i] = gload(Tj, rc + unsigned(c >= 0 ? c : 0) * n2);
+        }
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const int c = c1 - i;
+          if (c >= 0) {
+            const TX xc = TX(rlane(double(s * dinv), c));
+            if (lane < c) s -= TX(u[i]) * xc;
+            else if (lane == c) s = xc;
+          }
+        }
+      }
+      if (a.exch) s = __shfl(s, src);
+      if (row) x[size_t(j) * n + lane] = s;
     }
-    if (a.exch) s = __shfl(s, src);
-    if (row) x[size_t(j) * n + lane] = s;
+  } else {
+    for (int q0 = 0; q0 < nrhs; q0 += QB) {
+      TX vjp[QB], vjpp[QB], s[QB];
+      const TX* xq[QB];
+  #pragma unroll
+      for (int b = 0; b < QB; ++b) {
+        xq[b] = a.x + size_t(q0 + b < nrhs ? q0 + b : nrhs - 1) * a.ldx;
+        vjp[b] = row ? xq[b][size_t(jp) * n + lane] : TX(0);
+        vjpp[b] = row ? xq[b][size_t(jpp) * n + lane] : TX(0);
+        s[b] = row ? xq[b][size_t(j) * n + lane] : TX(0);
+      }
+      // D_j, E_j columns kChunk at a time (one memory latency per chunk)
+      for (int c0 = 0; c0 < n; c0 += kChunk) {
+        TW d[kChunk], e[kChunk];
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n;
+          d[i] = gload(Dj, rc + cc);
+          e[i] = gload(Ej, rc + cc);
+        }
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const int c = c0 + i;
+          if (c < n) {
+  #pragma unroll
+            for (int b = 0; b < QB; ++b)
+              s[b] -= TX(d[i]) * TX(rlane(double(vjp[b]), c)) + TX(e[i]) * TX(rlane(double(vjpp[b]), c));
+          }
+        }
+      }
+      // B_j xb, eight columns' loads in flight at a time
+      for (int c0 = 0; c0 < nx; c0 += 8) {
+        TW bv[8];
+  #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int cc = c0 + i < nx ? c0 + i : 0;
+          bv[i] = gload(Bj, rc + unsigned(cc) * n);
+        }
+  #pragma unroll
+        for (int b = 0; b < QB; ++b) {
+          TX xv[8];
+  #pragma unroll
+          for (int i = 0; i < 8; ++i) xv[i] = xq[b][xbrow + (c0 + i < nx ? c0 + i : 0)];
+  #pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if (c0 + i < nx) s[b] -= TX(keep_if(bv[i], row)) * xv[i];
+        }
+      }
+      // U^{-1}, bottom up: lane c owns the diagonal of column c; columns kChunk
+      // at a time from the top of the remaining range down
+      for (int c1 = n - 1; c1 >= 0; c1 -= kChunk) {
+        TW u[kChunk];
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const int c = c1 - i;
+          u[i] = gload(Tj, rc + unsigned(c >= 0 ? c : 0) * n2);
+        }
+  #pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+          const int c = c1 - i;
+          if (c >= 0) {
+  #pragma unroll
+            for (int b = 0; b < QB; ++b) {
+              const TX xc = TX(rlane(double(s[b] * dinv), c));
+              if (lane < c) s[b] -= TX(u[i]) * xc;
+              else if (lane == c) s[b] = xc;
+            }
+          }
+        }
+      }
+  #pragma unroll
+      for (int b = 0; b < QB; ++b) {
+        if (a.exch) s[b] = __shfl(s[b], src);
+        if (row && q0 + b < nrhs) a.x[size_t(q0 + b) * a.ldx + size_t(j) * n + lane] = s[b];
+      }
+    }
   }
 }
 
-template <typename TW, typename TX, bool FWD>
+template <typename TW, typename TX, bool FWD, int QB>
 __global__ void __launch_bounds__(256) sweep_lu_stream_kernel(SolveArgs<TW, TX> a) {
-  if (FWD) forward_lu_stream_body<TW, TX>(a, blockIdx.x);
-  else backward_lu_stream_body<TW, TX>(a, blockIdx.x);
+  if (FWD) forward_lu_stream_body<TW, TX, QB>(a, blockIdx.x);
+  else backward_lu_stream_body<TW, TX, QB>(a, blockIdx.x);
 }
 
 template <typename TW, typename TX, bool FWD>
 __global__ void __launch_bounds__(256) sweep_lu_stream_batched(BatchTable<SolveArgs<TW, TX>> t) {
   const int e = t.entry[blockIdx.x];
   if (skipped(t.args[e].skip)) return;
-  if (FWD) forward_lu_stream_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
-  else backward_lu_stream_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
+  if (FWD) forward_lu_stream_body<TW, TX, 1>(t.args[e], int(blockIdx.x) - t.first[e]);
+  else backward_lu_stream_body<TW, TX, 1>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
 }  // namespace
@@ -203,8 +298,14 @@ template <typename TW, typename TX>
 bool launch_sweep_lu_stream(bool forward, SolveArgs<TW, TX> const& a, hipStream_t st, hipError_t* err) {
   if (a.n < 1 || a.n > 64 || a.npairs < 1) return false;
   const int blocks = (a.npairs + 3) / 4;
-  if (forward) hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, true>), dim3(blocks), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, false>), dim3(blocks), dim3(256), 0, st, a);
+  // several right-hand sides: four per pass over the factors
+  if (a.nrhs > 1) {
+    if (forward) hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, true, 4>), dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, false, 4>), dim3(blocks), dim3(256), 0, st, a);
+  } else {
+    if (forward) hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, true, 1>), dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((sweep_lu_stream_kernel<TW, TX, false, 1>), dim3(blocks), dim3(256), 0, st, a);
+  }
   *err = hipGetLastError();
   return true;
 }
