@@ -116,8 +116,11 @@ struct MvArgs {
   const double* x;     // ncols
   const double* b;     // nrows (residual mode) or nullptr (y = A x)
   double* out;         // nrows
-  double* Cpart;       // [N+1][nr]
+  double* Cpart;       // [N+1][nr] (per column with nrhs > 1)
   const int32_t* skip = nullptr;   // batched: the system is done when *skip != 0
+  // several columns (residual mode): x, b, out advance by ldx, ldb, ldo
+  int nrhs = 1;
+  int64_t ldx = 0, ldb = 0, ldo = 0;
 };
 
 __host__ __device__ __forceinline__ size_t tsize(int n) { return size_t(2) * n * n + n; }

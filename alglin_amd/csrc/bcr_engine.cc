@@ -580,6 +580,26 @@ void Engine<TA, TW>::residual_async(double const* x, double const* b, double* r)
 }
 
 template <typename TA, typename TW>
+void Engine<TA, TW>::residual_multi_async(double const* x, int nrhs, int64_t ldx, double const* b, int64_t ldb,
+                                          double* r, int64_t ldr) {
+  if (nrhs <= 1) {
+    if (nrhs == 1) residual_async(x, b, r);
+    return;
+  }
+  if (!pD_) throw HipError("residual: no operator loaded or bound");
+  const size_t nr = size_t(std::max(1, d_.nr));
+  if (Cpart_.count < size_t(nrhs) * (d_.N + 1) * nr) Cpart_.resize(size_t(nrhs) * (d_.N + 1) * nr);
+  if (Csum_.count < size_t(nrhs) * nr) Csum_.resize(size_t(nrhs) * nr);
+  MvArgs<TA> m{};
+  m.N = d_.N; m.n = d_.n; m.qr = d_.qr; m.qx = d_.qx; m.nr = d_.nr; m.nx = d_.nx;
+  m.D = pD_; m.E = pE_; m.B = pB_; m.C = pC_; m.H = pH_; m.Cq = pCq_; m.F = pF_;
+  m.x = x; m.b = b; m.out = r; m.Cpart = Cpart_.p;
+  m.nrhs = nrhs; m.ldx = ldx; m.ldb = ldb; m.ldo = ldr;
+  tmpd_.resize(std::max<size_t>(tmpd_.count, size_t(max_chunks_) * std::max(1, d_.nr)));
+  timed(kResidual, [&] { hip_check(launch_mv(m, Csum_.p, tmpd_.p, max_chunks_, st_), "mv"); });
+}
+
+template <typename TA, typename TW>
 void Engine<TA, TW>::mv_async(double const* x, double* y) {
   residual_async(x, nullptr, y);
 }

@@ -123,6 +123,10 @@ class Engine {
 
   // r = b - A x (FP64, pristine operator); all device pointers
   void residual_async(double const* x, double const* b, double* r);
+  // R_q = B_q - A X_q for nrhs columns (X_q = x + q ldx, B_q, R_q likewise):
+  // one pass over the blocks for up to four columns
+  void residual_multi_async(double const* x, int nrhs, int64_t ldx, double const* b, int64_t ldb, double* r,
+                            int64_t ldr);
   // y = A x
   void mv_async(double const* x, double* y);
   // out = |A| |x| + |b| (FP64, pristine operator): the row scales of the

@@ -140,6 +140,110 @@ __device__ __forceinline__ void mv_rows_run(MvArgs<TA> const& a, int bid) {
   }
 }
 
+// r_q = b_q - A x_q for the nrhs columns q (x_q = x + q ldx, b_q, out_q
+// likewise; border partials of column q at Cpart + q (N+1) nr): the rows of
+// mv_rows_run with each block's loads shared by QB columns at a time
+template <typename TA, bool PACKED, int QB>
+__device__ __forceinline__ void mv_rows_multi_run(MvArgs<TA> const& a, int bid) {
+  constexpr int CH = 8;
+  const int lane = threadIdx.x & 63;
+  const int N = a.N, n = a.n, nx = a.nx, nr = a.nr;
+  const int g = PACKED ? mv_group(n, nr) : 64;
+  const int bpw = PACKED ? 64 / g : 1;
+  const int k0 = __builtin_amdgcn_readfirstlane((bid * 4 + int(threadIdx.x >> 6)) * bpw);
+  if (k0 > N) return;
+  const int sub = PACKED ? lane / g : 0, r = PACKED ? lane - sub * g : lane;
+  const int k = k0 + sub;
+  const bool live = sub < bpw && k <= N;
+  const int kc = live ? k : k0;
+  const int base = PACKED ? sub * g : 0;
+  auto bcast = [&](double v, int c) -> double {
+    if constexpr (PACKED) return __shfl(v, base + c, 64);
+    else return wave::rl(v, c);
+  };
+  const bool row = live && k < N && r < n;
+  const int kr = row ? k : k0;
+  const int rr = r < n ? r : 0;
+  const TA* Dk = a.D + size_t(k0) * n * n;
+  const TA* Ek = a.E + size_t(k0) * n * n;
+  const TA* Bk = a.B + size_t(k0) * n * nx;
+  const TA* Ck = a.C + size_t(k0) * nr * n;
+  const unsigned odk = unsigned(kr - k0) * unsigned(n * n) + rr, obk = unsigned(kr - k0) * unsigned(n * nx) + rr;
+  const bool crow = live && r < nr;
+  const unsigned ock = unsigned(kc - k0) * unsigned(nr * n) + (crow ? r : 0);
+  const size_t out = size_t(kr) * n + rr;
+  for (int q0 = 0; q0 < a.nrhs; q0 += QB) {
+    double xa[QB], xn[QB], acc[QB], accc[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const int q = q0 + b < a.nrhs ? q0 + b : a.nrhs - 1;
+      const double* x = a.x + size_t(q) * a.ldx;
+      xa[b] = r < n ? x[size_t(kc) * n + rr] : 0.0;
+      xn[b] = k0 < N ? (row ? x + size_t(k) * n : x + size_t(k0) * n)[n + rr] : 0.0;
+      acc[b] = 0.0;
+      accc[b] = 0.0;
+    }
+    if (k0 < N) {
+      for (int c0 = 0; c0 < n; c0 += CH) {
+        TA d[CH], e[CH], cv[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = c0 + i < n ? c0 + i : 0;
+          d[i] = gload(Dk, odk + unsigned(cc * n));
+          e[i] = gload(Ek, odk + unsigned(cc * n));
+          cv[i] = nr > 0 ? gload(Ck, ock + unsigned(cc * nr)) : TA(0);
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = c0 + i < n ? c0 + i : 0;
+#pragma unroll
+          for (int b = 0; b < QB; ++b) {
+            const double u = bcast(xa[b], cc), v = bcast(xn[b], cc);
+            acc[b] += keep_if(double(d[i]) * u + double(e[i]) * v, c0 + i < n);
+            accc[b] += keep_if(double(cv[i]) * u, c0 + i < n);
+          }
+        }
+      }
+      for (int c0 = 0; c0 < nx; c0 += CH) {
+        TA bb[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) bb[i] = gload(Bk, obk + unsigned((c0 + i < nx ? c0 + i : 0) * n));
+#pragma unroll
+        for (int b = 0; b < QB; ++b) {
+          const int q = q0 + b < a.nrhs ? q0 + b : a.nrhs - 1;
+          const double* xb = a.x + size_t(q) * a.ldx + size_t(N + 1) * n + a.qx;
+#pragma unroll
+          for (int i = 0; i < CH; ++i) acc[b] += keep_if(double(bb[i]) * xb[c0 + i < nx ? c0 + i : 0], c0 + i < nx);
+        }
+      }
+    } else if (nr > 0) {
+      // a wave that starts at block row N: the border partials C_N x_N only
+      for (int c0 = 0; c0 < n; c0 += CH) {
+        TA cv[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) cv[i] = gload(Ck, ock + unsigned((c0 + i < n ? c0 + i : 0) * nr));
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const int cc = c0 + i < n ? c0 + i : 0;
+#pragma unroll
+          for (int b = 0; b < QB; ++b) accc[b] += keep_if(double(cv[i]) * bcast(xa[b], cc), c0 + i < n);
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const int q = q0 + b;
+      if (q < a.nrhs) {
+        if (nr > 0 && crow) a.Cpart[size_t(q) * (N + 1) * nr + size_t(k) * nr + r] = accc[b];
+        if (row) {
+          const double bv = a.b ? a.b[size_t(q) * a.ldb + out] : 0.0;
+          a.out[size_t(q) * a.ldo + out] = a.b ? bv - acc[b] : acc[b];
+        }
+      }
+    }
+  }
+}
+
 template <typename TA, bool ABS = false>
 __device__ __forceinline__ void mv_rows_body(MvArgs<TA> const& a, int bid) {
   if (mv_rows_per_wave(a.n, a.nr) > 1) mv_rows_run<TA, true, ABS>(a, bid);
@@ -149,6 +253,12 @@ __device__ __forceinline__ void mv_rows_body(MvArgs<TA> const& a, int bid) {
 template <typename TA, bool ABS>
 __global__ void __launch_bounds__(256) mv_rows_kernel(MvArgs<TA> a) {
   mv_rows_body<TA, ABS>(a, blockIdx.x);
+}
+
+template <typename TA>
+__global__ void __launch_bounds__(256) mv_rows_multi_kernel(MvArgs<TA> a) {
+  if (mv_rows_per_wave(a.n, a.nr) > 1) mv_rows_multi_run<TA, true, 4>(a, blockIdx.x);
+  else mv_rows_multi_run<TA, false, 4>(a, blockIdx.x);
 }
 
 
@@ -268,6 +378,27 @@ template <typename TA>
 hipError_t launch_mv(MvArgs<TA> const& a, double* Csum, double* tmp, int max_chunks, hipStream_t st, bool absval) {
   const int rows_per_block = 4 * mv_rows_per_wave(a.n, a.nr);
   const dim3 grid((a.N + 1 + rows_per_block - 1) / rows_per_block);
+  if (a.nrhs > 1 && !absval) {
+    // several columns: one pass over the blocks for up to four of them, then
+    // per column the border sums and the boundary / border rows (Cpart and
+    // Csum hold nrhs column slices)
+    hipLaunchKernelGGL((mv_rows_multi_kernel<TA>), grid, dim3(256), 0, st, a);
+    for (int q = 0; q < a.nrhs; ++q) {
+      MvArgs<TA> c = a;
+      c.nrhs = 1;
+      c.x = a.x + size_t(q) * a.ldx;
+      c.b = a.b ? a.b + size_t(q) * a.ldb : nullptr;
+      c.out = a.out + size_t(q) * a.ldo;
+      c.Cpart = a.Cpart + size_t(q) * (a.N + 1) * a.nr;
+      double* cs = Csum + size_t(q) * a.nr;
+      if (a.nr > 0) {
+        launch_convert<double, double>(nullptr, cs, a.nr, st);
+        launch_reduce<double, double>(c.Cpart, 0, a.N + 1, a.nr, tmp, max_chunks, cs, a.nr, a.nr, st);
+      }
+      hipLaunchKernelGGL((mv_tail_kernel<TA, false>), dim3(1), dim3(256), 0, st, c, cs);
+    }
+    return hipGetLastError();
+  }
   if (absval) hipLaunchKernelGGL((mv_rows_kernel<TA, true>), grid, dim3(256), 0, st, a);
   else hipLaunchKernelGGL((mv_rows_kernel<TA, false>), grid, dim3(256), 0, st, a);
   if (a.nr > 0) {

@@ -86,13 +86,15 @@ RefineReport MixedEngine::solve(double* X, int nrhs, int64_t ld, double tol, int
   std::vector<double> best(static_cast<size_t>(nrhs), INFINITY);
   std::vector<int> stagnant(static_cast<size_t>(nrhs), 0);
   for (int it = 0;; ++it) {
-    // residuals of the open columns: column open[i] -> slot i of r_
+    // residuals of every column in one pass over the blocks (column c ->
+    // slot c of r_; a closed column's x no longer changes, so computing it
+    // costs flops, not bytes); the norms of the open ones, open[i] -> 2i
     const int m = int(open.size());
+    eng_.residual_multi_async(X, nrhs, ld, b_.p, L, r_.p, L);
     for (int i = 0; i < m; ++i) {
       const int c = open[i];
-      eng_.residual_async(X + size_t(c) * ld, b_.p + size_t(c) * L, r_.p + size_t(i) * L);
       eng_.nrm2_async(b_.p + size_t(c) * L, nrows, norms_.p + 2 * i);
-      eng_.nrm2_async(r_.p + size_t(i) * L, nrows, norms_.p + 2 * i + 1);
+      eng_.nrm2_async(r_.p + size_t(c) * L, nrows, norms_.p + 2 * i + 1);
     }
     hip_check(hipMemcpyAsync(hnorm_.data(), norms_.p, size_t(2) * m * sizeof(double), hipMemcpyDeviceToHost, st),
               "norms");
@@ -117,7 +119,7 @@ RefineReport MixedEngine::solve(double* X, int nrhs, int64_t ld, double tol, int
         // backward error tells whether FP64 can get any further (the floor),
         // and three such sweeps in a row end the column as stalled
         ++stagnant[c];
-        const double om = backward_error(X + size_t(c) * ld, b_.p + size_t(c) * L, r_.p + size_t(i) * L);
+        const double om = backward_error(X + size_t(c) * ld, b_.p + size_t(c) * L, r_.p + size_t(c) * L);
         if (!(om <= rep.backward_error)) rep.backward_error = om;
         if (om <= kFloorBerr) done = kFloor;
         else if (stagnant[c] >= 3) done = kStalled;
@@ -130,17 +132,18 @@ RefineReport MixedEngine::solve(double* X, int nrhs, int64_t ld, double tol, int
     }
     if (keep.empty()) break;
     // the residuals of the columns still open become the correction
-    // right-hand sides, packed to the front of r_ (slot keep[k] -> k, k <=
-    // keep[k], so no slot is overwritten before it is read); rows beyond
-    // nrows (when ncols > nrows) are zero
+    // right-hand sides, packed to the front of r_ (slot open[keep[k]] -> k;
+    // the open columns ascend, so k <= open[keep[k]] and no slot is
+    // overwritten before it is read); rows beyond nrows (when ncols > nrows)
+    // are zero
     std::vector<int> next;
     for (int k = 0; k < int(keep.size()); ++k) {
-      const int i = keep[k];
-      if (i != k)
-        hip_check(launch_convert<double, double>(r_.p + size_t(i) * L, r_.p + size_t(k) * L, nrows, st), "pack r");
+      const int c = open[keep[k]];
+      if (c != k)
+        hip_check(launch_convert<double, double>(r_.p + size_t(c) * L, r_.p + size_t(k) * L, nrows, st), "pack r");
       if (L > nrows)
         hip_check(launch_convert<double, double>(nullptr, r_.p + size_t(k) * L + nrows, L - nrows, st), "pad");
-      next.push_back(open[i]);
+      next.push_back(c);
     }
     const int k = int(next.size());
     eng_.solve_async<double>(r_.p, k, L);
