@@ -456,7 +456,7 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
   hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
   hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
   // the n > 32 class on st2_ unless every launch is timed (then one stream)
-  const bool conc = ev == nullptr;
+  const bool conc = ev == nullptr && streams_ > 1;
   if (conc) fork();
   size_t li = 0;
   for (int k = 0; k < levels_; ++k)
@@ -626,9 +626,10 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
   const int ns = int(active_.size());
   if (ns == 0) return;
   // the n > 32 class's sweeps on st2_, concurrent with the other classes'
+  const bool conc = streams_ > 1;
   auto sweep = [&](Launch const& l, bool fwd) {
     BatchTable<SolveArgs<TW, double>> t{static_cast<const SolveArgs<TW, double>*>(l.args), l.entry, l.first};
-    hipStream_t s = class_stream(l.cls, true);
+    hipStream_t s = class_stream(l.cls, conc);
     hipError_t e = hipSuccess;
     if (l.cls <= 2 || (l.cls == 3 && fwd)) {
       if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, s, &e))
@@ -641,16 +642,16 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
     }
     hip_check(e, fwd ? "batch forward" : "batch backward");
   };
-  fork();
+  if (conc) fork();
   for (int k = 0; k < levels_; ++k)
     for (Launch const& l : T.fwd[k]) sweep(l, true);
-  join();
+  if (conc) join();
   hip_check(launch_reduce_batched<double, double>(T.xb, ns, st_), "reduce xb");
   hip_check(launch_last_solve_batched<double>(T.last, ns, Nrmax_, Ncmax_, lds_limit_, st_), "last solve");
-  fork();
+  if (conc) fork();
   for (int k = levels_ - 1; k >= 0; --k)
     for (Launch const& l : T.fwd[k]) sweep(l, false);
-  join();
+  if (conc) join();
 }
 
 template <typename TA, typename TW>
@@ -927,6 +928,13 @@ void BatchSolver::set_pair_variant(int v) {
   drop_graphs();
   if (e32_) e32_->set_pair_variant(v);
   if (e64_) e64_->set_pair_variant(v);
+}
+
+void BatchSolver::set_streams(int k) {
+  drop_graphs();
+  streams_ = k;
+  if (e32_) e32_->set_streams(k);
+  if (e64_) e64_->set_streams(k);
 }
 
 void BatchSolver::set_class_kernel(bool fp32_factors, int cls, int kernel, int waves) {
@@ -1256,6 +1264,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       e64_->plan(dims_);
       e64_->bind_device(oD_, oE_, oB_, oC_, oH_, oCq_, oF_);
       apply_tuning(*e64_);
+      e64_->set_streams(streams_);
     }
     auto& f = *e64_;
     hip_check(hipStreamSynchronize(st), "sync before fallback");

@@ -768,6 +768,14 @@ int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant) {
   });
 }
 
+int alglin_batch_set_streams(alglin_batch_t h, int streams) {
+  return bguard(h, [&] {
+    if (streams != 1 && streams != 2) throw std::invalid_argument("streams: 1 or 2");
+    h->s->set_streams(streams);
+    return 0;
+  });
+}
+
 int alglin_batch_set_class_kernel(alglin_batch_t h, int fp32_factors, int cls, int kernel, int waves) {
   return bguard(h, [&] {
     h->s->set_class_kernel(fp32_factors != 0, cls, kernel, waves);
