@@ -83,7 +83,7 @@ def lib():
             "alglin_memcpy_dtod_async": (ci, [vp, vp, i64, vp]),
             "alglin_batch_time_factorize": (ci, [vp, vp, i32]),
             "alglin_batch_set_pair_kernel": (ci, [vp, ctypes.c_int]),
-            "alglin_batch_set_streams": (ci, [vp, ctypes.c_int]),
+            "alglin_batch_set_streams": (ci, [vp, ctypes.c_int, ctypes.c_int]),
             "alglin_batch_set_class_kernel": (ci, [vp, ci, ci, ci, ci]),
             "alglin_batch_create": (vp, [ctypes.c_int]),
             "alglin_batch_destroy": (None, [vp]),
@@ -626,10 +626,11 @@ class Batch:
         column-split), 5 the matrix-core pair."""
         self._check(lib().alglin_batch_set_pair_kernel(self._h, int(variant)), "set_pair_kernel")
 
-    def set_streams(self, streams: int):
+    def set_streams(self, streams: int, fork_level: int = 0):
         """1: every launch on the batch's stream; 2 (default): the n > 32 LU
-        class on a second stream beside the other classes."""
-        self._check(lib().alglin_batch_set_streams(self._h, int(streams)), "set_streams")
+        class on a second stream beside the other classes, from reduction
+        level fork_level on."""
+        self._check(lib().alglin_batch_set_streams(self._h, int(streams), int(fork_level)), "set_streams")
 
     def set_class_kernel(self, cls: int, kernel: int, waves: int = 0, fp32_factors=None):
         """Kernel of LU pair class cls (3: 16 < n <= 32, 4: 32 < n <= 64):

@@ -455,13 +455,18 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
   hip_check(launch_convert<int32_t, int32_t>(nullptr, skip_.p, nsys(), st_), "skip flags");
   hip_check(launch_convert<TW, TW>(nullptr, Cpend_.p, int64_t(Cpend_.count), st_), "zero pending");
   hip_check(launch_prep_batched<TA, TW>(prep_, ns, st_), "prep");
-  // the n > 32 class on st2_ unless every launch is timed (then one stream)
-  const bool conc = ev == nullptr && streams_ > 1;
-  if (conc) fork();
+  // the n > 32 class on st2_ from level fork_level_ on, unless every launch
+  // is timed (then one stream)
+  const bool conc = ev == nullptr && streams_ > 1 && fork_level_ < levels_;
+  bool forked = false;
   size_t li = 0;
   for (int k = 0; k < levels_; ++k)
     for (Launch const& l : fac_[k]) {
-      hipStream_t s = class_stream(l.cls, conc);
+      if (conc && !forked && k >= fork_level_) {
+        fork();
+        forked = true;
+      }
+      hipStream_t s = class_stream(l.cls, forked);
       if (ev) hip_check(hipEventRecord((*ev)[2 * li], s), "event");
       hipError_t e = hipSuccess;
       auto run = [&](auto const* args) {
@@ -495,7 +500,7 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
       if (ev) hip_check(hipEventRecord((*ev)[2 * li + 1], s), "event");
       ++li;
     }
-  if (conc) join();
+  if (forked) join();
   hip_check(launch_convert<TA, double>(pF_, Feff_.p, tot_[kbF], st_), "F");
   hip_check(launch_reduce_batched<TW, double>(fred_, ns, st_), "reduce F");
   hip_check(launch_assemble_last_batched<TW, TA>(asm_, ns, st_), "assemble last");
@@ -626,10 +631,11 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
   const int ns = int(active_.size());
   if (ns == 0) return;
   // the n > 32 class's sweeps on st2_, concurrent with the other classes'
-  const bool conc = streams_ > 1;
+  const bool conc = streams_ > 1 && fork_level_ < levels_;
+  bool forked = false;
   auto sweep = [&](Launch const& l, bool fwd) {
     BatchTable<SolveArgs<TW, double>> t{static_cast<const SolveArgs<TW, double>*>(l.args), l.entry, l.first};
-    hipStream_t s = class_stream(l.cls, conc);
+    hipStream_t s = class_stream(l.cls, forked);
     hipError_t e = hipSuccess;
     if (l.cls <= 2 || (l.cls == 3 && fwd)) {
       if (!launch_sweep_lu_fast_batched(fwd, t, l.nblocks, l.nmax, 1, s, &e))
@@ -642,16 +648,31 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
     }
     hip_check(e, fwd ? "batch forward" : "batch backward");
   };
-  if (conc) fork();
-  for (int k = 0; k < levels_; ++k)
+  // forward: the levels before fork_level_ on one stream, then two; backward
+  // (levels in reverse): two streams down to fork_level_, then one
+  for (int k = 0; k < levels_; ++k) {
+    if (conc && !forked && k >= fork_level_) {
+      fork();
+      forked = true;
+    }
     for (Launch const& l : T.fwd[k]) sweep(l, true);
-  if (conc) join();
+  }
+  if (forked) join();
+  forked = false;
   hip_check(launch_reduce_batched<double, double>(T.xb, ns, st_), "reduce xb");
   hip_check(launch_last_solve_batched<double>(T.last, ns, Nrmax_, Ncmax_, lds_limit_, st_), "last solve");
-  if (conc) fork();
-  for (int k = levels_ - 1; k >= 0; --k)
+  if (conc && fork_level_ < levels_) {
+    fork();
+    forked = true;
+  }
+  for (int k = levels_ - 1; k >= 0; --k) {
+    if (forked && k < fork_level_) {
+      join();
+      forked = false;
+    }
     for (Launch const& l : T.fwd[k]) sweep(l, false);
-  if (conc) join();
+  }
+  if (forked) join();
 }
 
 template <typename TA, typename TW>
@@ -930,11 +951,12 @@ void BatchSolver::set_pair_variant(int v) {
   if (e64_) e64_->set_pair_variant(v);
 }
 
-void BatchSolver::set_streams(int k) {
+void BatchSolver::set_streams(int k, int fork_level) {
   drop_graphs();
   streams_ = k;
-  if (e32_) e32_->set_streams(k);
-  if (e64_) e64_->set_streams(k);
+  fork_level_ = fork_level;
+  if (e32_) e32_->set_streams(k, fork_level);
+  if (e64_) e64_->set_streams(k, fork_level);
 }
 
 void BatchSolver::set_class_kernel(bool fp32_factors, int cls, int kernel, int waves) {
@@ -1264,7 +1286,7 @@ BatchReport BatchSolver::solve_selected(double const* B, double* X, double tol, 
       e64_->plan(dims_);
       e64_->bind_device(oD_, oE_, oB_, oC_, oH_, oCq_, oF_);
       apply_tuning(*e64_);
-      e64_->set_streams(streams_);
+      e64_->set_streams(streams_, fork_level_);
     }
     auto& f = *e64_;
     hip_check(hipStreamSynchronize(st), "sync before fallback");

@@ -127,8 +127,13 @@ class BatchEngine {
   // 5 the matrix-core pair
   void set_pair_variant(int v) { pair_variant_ = v; }
   // factorization and sweeps on one stream (1) or with the n > 32 class on
-  // a second stream beside the others (2, the default)
-  void set_streams(int k) { streams_ = k > 1 ? 2 : 1; }
+  // a second stream beside the others (2, the default), the second stream
+  // forked from reduction level `fork_level` on (the levels before it run
+  // one launch after the other)
+  void set_streams(int k, int fork_level = 0) {
+    streams_ = k > 1 ? 2 : 1;
+    fork_level_ = fork_level < 0 ? 0 : fork_level;
+  }
   int streams() const { return streams_; }
 
   // one eager factorization with HIP events around every pair-elimination
@@ -198,7 +203,7 @@ class BatchEngine {
   hipEvent_t fork_ = nullptr, join_ = nullptr;
   void fork();
   void join();
-  int streams_ = 2;
+  int streams_ = 2, fork_level_ = 0;
   hipStream_t class_stream(int cls, bool concurrent) const { return concurrent && cls == 4 ? st2_ : st_; }
 
   TA const *pD_ = nullptr, *pE_ = nullptr, *pB_ = nullptr, *pC_ = nullptr, *pH_ = nullptr, *pCq_ = nullptr,
@@ -291,8 +296,9 @@ class BatchSolver {
   // kernel of the 32 < n <= 64 LU class (BatchEngine::set_pair_variant)
   void set_pair_variant(int v);
   // 1: every launch on the batch's stream; 2 (default): the n > 32 LU class
-  // on a second stream (drops the captured graphs)
-  void set_streams(int k);
+  // on a second stream from reduction level fork_level on (drops the
+  // captured graphs)
+  void set_streams(int k, int fork_level);
   // tuned kernel of an LU pair class for the FP32-factor engine or the FP64
   // one (BatchEngine::set_class_kernel); kept for an engine created later
   void set_class_kernel(bool fp32_factors, int cls, int kernel, int waves);
@@ -312,7 +318,7 @@ class BatchSolver {
   std::vector<char> floor_;          // per system: stopped at the FP64 residual floor
   DevBuf<double> den_, berr_;        // |A||x| + |b| rows, per-system backward errors
   bool graphs_ = true;   // solve_selected replays captured launch lists
-  int streams_ = 2;
+  int streams_ = 2, fork_level_ = 0;
   int tune32_[kFacClasses][2] = {{-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}};
   int tune64_[kFacClasses][2] = {{-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}, {-1, 0}};
   void apply_tuning(BatchEngine<double, double>& e);

@@ -768,10 +768,11 @@ int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant) {
   });
 }
 
-int alglin_batch_set_streams(alglin_batch_t h, int streams) {
+int alglin_batch_set_streams(alglin_batch_t h, int streams, int fork_level) {
   return bguard(h, [&] {
     if (streams != 1 && streams != 2) throw std::invalid_argument("streams: 1 or 2");
-    h->s->set_streams(streams);
+    if (fork_level < 0) throw std::invalid_argument("fork_level >= 0");
+    h->s->set_streams(streams, fork_level);
     return 0;
   });
 }

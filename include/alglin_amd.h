@@ -271,9 +271,10 @@ int alglin_batch_set_pair_kernel(alglin_batch_t h, int variant);
 
 /* 1: every launch of the batch on its stream, one after the other; 2 (the
    default): the 32 < n <= 64 LU class factorized and swept on a second
-   stream beside the other classes.  Drops captured graphs.  No counterpart
-   in the reference (a tuning hook). */
-int alglin_batch_set_streams(alglin_batch_t h, int streams);
+   stream beside the other classes, from reduction level fork_level on (the
+   levels before it one launch after the other).  Drops captured graphs.  No
+   counterpart in the reference (a tuning hook). */
+int alglin_batch_set_streams(alglin_batch_t h, int streams, int fork_level);
 
 /* Tuned kernel of one LU pair-elimination class (no counterpart in the
    reference: the hook the autotuner, alglin_amd/autotune.py, drives).
