@@ -178,3 +178,21 @@ def test_selected_fallback_stages_reach_tolerance():
     assert (st >= 1).sum() > 0
     # the systems that need the fallback are mostly the nearly singular ones
     assert (st[sing >= 1] >= 1).sum() > (st[sing < 1] >= 1).sum()
+
+
+@pytest.mark.parametrize("streams,fork_level", [(1, 0), (2, 1)])
+def test_batch_streams_give_the_same_bits(streams, fork_level):
+    """The stream layout (one stream, or the n > 32 class on a second stream
+    from a given level) changes only the order in which independent systems'
+    launches run: the mixed solution is bit-identical to the default's."""
+    dims = mixed_dims()
+    print(f"case batch streams={streams} fork_level={fork_level}", flush=True)
+    bt = Batch(mixed=True)
+    bt.plan(dims)
+    bt.generate(seed=9, shift=4.0)
+    B = DeviceArray.from_host(random_rhs(bt, 2))
+    X0, X1 = DeviceArray(bt.size("x")), DeviceArray(bt.size("x"))
+    assert bt.solve(B, X0, tol=1e-12, max_iter=30, graph=True) == 0, bt.report()
+    bt.set_streams(streams, fork_level)
+    assert bt.solve(B, X1, tol=1e-12, max_iter=30, graph=True) == 0, bt.report()
+    assert np.array_equal(X0.to_host(), X1.to_host())
