@@ -139,6 +139,7 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
   for (auto* v : {&offT_, &offPerm_, &offCpend_, &offFpart_, &offHm_, &offPiv_, &offXpart_, &offCpart_, &offCsum_})
     v->assign(ns + 1, 0);
   maxN_ = Ncmax_ = Nrmax_ = 0;
+  square_ = true;
   for (int s = 0; s < ns; ++s) {
     SysDims const& d = dims_[s];
     const int64_t N = d.N, n = d.n, nr = d.nr, nx = d.nx;
@@ -161,6 +162,7 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
     maxN_ = std::max(maxN_, d.N);
     Ncmax_ = std::max(Ncmax_, d.Nc());
     Nrmax_ = std::max(Nrmax_, d.Nr());
+    square_ = square_ && d.Nr() == d.Nc();
   }
   for (int k = 0; k < kbKinds; ++k) tot_[k] = off_[k][ns];
   levels_ = levels_of(maxN_);
@@ -660,7 +662,12 @@ void BatchEngine<TA, TW>::solve_async(double* X) {
   if (forked) join();
   forked = false;
   hip_check(launch_reduce_batched<double, double>(T.xb, ns, st_), "reduce xb");
-  hip_check(launch_last_solve_batched<double>(T.last, ns, Nrmax_, Ncmax_, lds_limit_, st_), "last solve");
+  // the last blocks are LU-factored with one right-hand side here: one
+  // wavefront per system when they are square and Nc <= 256
+  hipError_t le = hipSuccess;
+  if (!(square_ && launch_last_solve_lu_wave_batched<double>(T.last, ns, Ncmax_, st_, &le)))
+    le = launch_last_solve_batched<double>(T.last, ns, Nrmax_, Ncmax_, lds_limit_, st_);
+  hip_check(le, "last solve");
   if (conc && fork_level_ < levels_) {
     fork();
     forked = true;

@@ -192,6 +192,7 @@ class BatchEngine {
   // factor-side offsets
   std::vector<int64_t> offT_, offPerm_, offCpend_, offFpart_, offHm_, offPiv_, offXpart_, offCpart_, offCsum_;
   int maxN_ = 0, levels_ = 0, Ncmax_ = 0, Nrmax_ = 0;
+  bool square_ = true;   // every last block square (Nr == Nc)
   size_t lds_limit_ = 65536, factor_bytes_ = 0;
   int cls_kernel_[kFacClasses] = {-1, -1, -1, -1, -1, -1, -1};
   int cls_waves_[kFacClasses] = {0, 0, 0, 0, 0, 0, 0};

@@ -158,6 +158,11 @@ hipError_t launch_lu_last_batched(const LastSolveArgs<double>* args, int32_t* co
 template <typename TX>
 hipError_t launch_last_solve_batched(const LastSolveArgs<TX>* args, int nsys, int Nrmax, int Ncmax, size_t lds_limit,
                                      hipStream_t st);
+// one wavefront per system: square last blocks, LU, one right-hand side
+// (Ncmax <= 256; false when it does not apply)
+template <typename TX>
+bool launch_last_solve_lu_wave_batched(const LastSolveArgs<TX>* args, int nsys, int Ncmax, hipStream_t st,
+                                       hipError_t* err);
 
 // residual SpMV
 template <typename TA>

@@ -294,6 +294,9 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
     }
   }
   int pos0 = lane, pos1 = lane + 64;   // positions of rows lane, lane + 64
+  // 2n <= 64: every pivot candidate sits in row set 0 (row set 1 holds
+  // border rows only), and the pivot row's entries come from it unselected
+  constexpr bool ONE = 2 * NMAX <= 64;
   int myperm = 0;
   bool zero_pivot = false;
   // Steps in blocks of 16: step c of block b reads and rewrites column c
@@ -309,7 +312,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
       const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
       const TW x2 = RS > 2 ? V[RS - 1][b][cc] : TW(0);   // border rows (row set 2)
       const bool c0 = lane < n2 && pos0 >= c;
-      const bool c1 = lane + 64 < n2 && pos1 >= c;
+      const bool c1 = !ONE && lane + 64 < n2 && pos1 >= c;
       unsigned h0, l0, h1, l1;
       wave::abs_key(x0, h0, l0);
       wave::abs_key(x1, h1, l1);
@@ -319,17 +322,17 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
       const int who = wave::argmax_pos<(sizeof(TW) > 4)>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane,
                                                           zero);
       const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
-      const bool sp = __builtin_amdgcn_readlane(int(b1), who) != 0;
+      const bool sp = !ONE && __builtin_amdgcn_readlane(int(b1), who) != 0;
       zero_pivot = zero_pivot || zero;
       myperm = lane == c ? P + 1 : myperm;
       // the pivot row moves to position c, the row at c to the pivot's old place
       pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
-      pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+      if constexpr (!ONE) pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
       const TW pv = wave::rl(sp ? x1 : x0, who);
       const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
       // rows below the pivot get the multiplier, the others keep column c
       const bool u0 = lane < n2 ? pos0 > c : lane < R;
-      const bool u1 = lane + 64 < n2 ? pos1 > c : lane + 64 < R;
+      const bool u1 = !ONE && lane + 64 < n2 ? pos1 > c : lane + 64 < R;
       const TW m0 = x0 * rinv, m1 = x1 * rinv;
       const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
       V[0][b][cc] = u0 ? m0 : x0;

@@ -16,6 +16,7 @@
 // reference's swap of x[0:n] with x[(N-1)n:Nn], cc:1392-1423, is the same
 // gather/scatter written in place).
 #include "bcr_launch.hh"
+#include "bcr_wave.hh"
 
 namespace bcr {
 
@@ -284,6 +285,104 @@ __global__ void __launch_bounds__(1024) last_solve_batched(const LastSolveArgs<T
   last_solve_body<TX, A_LDS>(args[blockIdx.x]);
 }
 
+// The LU solve of a square last block for one right-hand side, one
+// wavefront per system: lane l holds rows l + 64 k (k < RPL) of y, the row
+// interchanges are composed into a source row per lane (no serial swaps),
+// and the two substitutions walk the columns of the factors from HBM/L2
+// eight at a time with y[c] broadcast by v_readlane, without a workgroup
+// barrier.  Same arithmetic per element as last_solve_body's LU path.
+template <typename TX, int RPL>
+__global__ void __launch_bounds__(64) last_solve_lu_wave_batched(const LastSolveArgs<TX>* args) {
+  const LastSolveArgs<TX>& a = args[blockIdx.x];
+  if (skipped(a.skip)) return;
+  constexpr int CH = 8;
+  const int lane = threadIdx.x;
+  const int n = a.n, Nc = a.Nc, ld = a.Nr;
+  const double* A = a.Hm;
+  const size_t xN = size_t(a.N) * n, xC = size_t(a.N + 1) * n + a.qr;
+  TX* const x = a.x;
+  // the source row of each row after the interchanges c = 0 .. Nc-1
+  // (1-based ipiv), found by undoing them from the last
+  int pos[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) pos[k] = lane + 64 * k;
+  for (int c = Nc - 1; c >= 0; --c) {
+    const int pv = a.P[c] - 1;
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) pos[k] = pos[k] == c ? pv : pos[k] == pv ? c : pos[k];
+  }
+  double y[RPL], dinv[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k, s = r < Nc ? pos[k] : 0;
+    const size_t row = s < n ? size_t(s) : s < 2 * n + a.qr ? xN + (s - n) : xC + (s - 2 * n - a.qr);
+    y[k] = r < Nc ? double(x[row]) : 0.0;
+    dinv[k] = r < Nc ? 1.0 / A[r + size_t(r) * ld] : 1.0;
+  }
+  // y <- L^{-1} y (unit lower), column c of L below the diagonal
+#pragma unroll
+  for (int kc = 0; kc < RPL; ++kc) {
+    const int cend = min(64 * kc + 64, Nc);
+    for (int c0 = 64 * kc; c0 < cend; c0 += CH) {
+      double l[CH][RPL];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const int c = min(c0 + i, Nc - 1), r = lane + 64 * k;
+          l[i][k] = A[(r < Nc ? r : 0) + size_t(c) * ld];
+        }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int c = c0 + i;
+        if (c < cend) {
+          const double yc = wave::rl(y[kc], c - 64 * kc);
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            const int r = lane + 64 * k;
+            if (r > c && r < Nc) y[k] -= l[i][k] * yc;
+          }
+        }
+      }
+    }
+  }
+  // y <- U^{-1} y, bottom up: y_c = y_c / u_cc, then the rows above
+#pragma unroll
+  for (int kc = RPL - 1; kc >= 0; --kc) {
+    const int cbeg = 64 * kc;
+    for (int c1 = min(64 * kc + 63, Nc - 1); c1 >= cbeg; c1 -= CH) {
+      double u[CH][RPL];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const int c = max(c1 - i, 0), r = lane + 64 * k;
+          u[i][k] = A[(r < Nc ? r : 0) + size_t(c) * ld];
+        }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        const int c = c1 - i;
+        if (c >= cbeg) {
+          const double xc = wave::rl(y[kc] * dinv[kc], c - 64 * kc);
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            const int r = lane + 64 * k;
+            y[k] = r < c ? y[k] - u[i][k] * xc : r == c ? xc : y[k];
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    if (r < Nc) {
+      const size_t row = r < n ? size_t(r) : r < 2 * n ? xN + (r - n) : size_t(a.N + 1) * n + (r - 2 * n);
+      x[row] = TX(y[k]);
+    }
+  }
+}
+
 template <typename TW, typename TA>
 hipError_t launch_assemble_last_batched(const AssembleArgs<TW, TA>* args, int nsys, hipStream_t st) {
   hipLaunchKernelGGL((assemble_last_batched<TW, TA>), dim3(nsys), dim3(256), 0, st, args);
@@ -319,6 +418,21 @@ hipError_t launch_last_solve_batched(const LastSolveArgs<TX>* args, int nsys, in
 template hipError_t launch_assemble_last_batched<float, double>(const AssembleArgs<float, double>*, int, hipStream_t);
 template hipError_t launch_assemble_last_batched<double, double>(const AssembleArgs<double, double>*, int, hipStream_t);
 template hipError_t launch_last_solve_batched<double>(const LastSolveArgs<double>*, int, int, int, size_t, hipStream_t);
+// the wavefront LU solve above, for square last blocks with Nc <= 256 solved
+// by LU for one right-hand side; false when it does not apply
+template <typename TX>
+bool launch_last_solve_lu_wave_batched(const LastSolveArgs<TX>* args, int nsys, int Ncmax, hipStream_t st,
+                                       hipError_t* err) {
+  if (Ncmax < 1 || Ncmax > 256) return false;
+  if (Ncmax <= 64) hipLaunchKernelGGL((last_solve_lu_wave_batched<TX, 1>), dim3(nsys), dim3(64), 0, st, args);
+  else if (Ncmax <= 128) hipLaunchKernelGGL((last_solve_lu_wave_batched<TX, 2>), dim3(nsys), dim3(64), 0, st, args);
+  else hipLaunchKernelGGL((last_solve_lu_wave_batched<TX, 4>), dim3(nsys), dim3(64), 0, st, args);
+  *err = hipGetLastError();
+  return true;
+}
+template bool launch_last_solve_lu_wave_batched<double>(const LastSolveArgs<double>*, int, int, hipStream_t,
+                                                        hipError_t*);
+
 
 template <typename TW, typename TA>
 hipError_t launch_assemble_last(AssembleArgs<TW, TA> const& a, hipStream_t st) {

@@ -74,9 +74,16 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
   {
     const int pvl = (live && r < n) ? a.perm[size_t(j) * n + r] - 1 : r;
     for (int c = 0; c < n; ++c) {
-      const int pv = __shfl(pvl, base + c, 64);
-      const int sc = __shfl(src, base + c, 64);
-      const int sp = __shfl(src, base + pv, 64);
+      int pv, sc, sp;
+      if constexpr (G == 64) {   // one pair per wavefront: c and pv are uniform
+        pv = __builtin_amdgcn_readlane(pvl, c);
+        sc = __builtin_amdgcn_readlane(src, c);
+        sp = __builtin_amdgcn_readlane(src, pv);
+      } else {
+        pv = __shfl(pvl, base + c, 64);
+        sc = __shfl(src, base + c, 64);
+        sp = __shfl(src, base + pv, 64);
+      }
       src = (r == c) ? sp : ((r == pv) ? sc : src);
     }
   }
