@@ -161,15 +161,26 @@ __device__ __forceinline__ double rl_after(double v, int who, int cc) {
 // columns K.. 15 of one 16-column block of the row sets, those not after
 // column cc of the block getting a zero pivot-row entry; the pivot row is in
 // row set 0 or 1 (a third row set holds border rows only)
-template <int K, int RS, typename TW>
+// f(integral_constant<int, I>) for I = B .. E-1, unrolled at compile time
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Columns before KM may lie at or before cc and are masked; the later ones
+// always lie after it.
+template <int K, int KM, int RS, typename TW>
 __device__ __forceinline__ void update_after(V16<TW>& v0, V16<TW>& v1, V16<TW>& v2, bool sp, int who, int cc, TW lv0,
                                              TW lv1, TW lv2) {
   if constexpr (K < 16) {
-    const TW u = rl_after<K>(sp ? v1[K] : v0[K], who, cc);
+    const TW u = K < KM ? rl_after<K>(sp ? v1[K] : v0[K], who, cc) : wave::rl(sp ? v1[K] : v0[K], who);
     v0[K] = fma(-lv0, u, v0[K]);
     v1[K] = fma(-lv1, u, v1[K]);
     if constexpr (RS > 2) v2[K] = fma(-lv2, u, v2[K]);
-    update_after<K + 1, RS, TW>(v0, v1, v2, sp, who, cc, lv0, lv1, lv2);
+    update_after<K + 1, KM, RS, TW>(v0, v1, v2, sp, who, cc, lv0, lv1, lv2);
   }
 }
 
@@ -299,63 +310,80 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
   constexpr bool ONE = 2 * NMAX <= 64;
   int myperm = 0;
   bool zero_pivot = false;
-  // Steps in blocks of 16: step c of block b reads and rewrites column c
-  // through a uniform register index (s_set_gpr_idx) and updates the columns
-  // of blocks b.. with the pivot row's entries, those of block b not after c
-  // getting a zero entry.  The loop stays rolled: a few KB of code instead of
-  // one copy per step.
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int cend = min(16 * b + 16, n);
-    for (int c = 16 * b; c < cend; ++c) {
-      const int cc = c - 16 * b;
-      const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
-      const TW x2 = RS > 2 ? V[RS - 1][b][cc] : TW(0);   // border rows (row set 2)
-      const bool c0 = lane < n2 && pos0 >= c;
-      const bool c1 = !ONE && lane + 64 < n2 && pos1 >= c;
-      unsigned h0, l0, h1, l1;
-      wave::abs_key(x0, h0, l0);
-      wave::abs_key(x1, h1, l1);
-      // the better of this lane's two candidates (ties to the smaller position)
-      const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
-      bool zero;
-      const int who = wave::argmax_pos<(sizeof(TW) > 4)>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane,
-                                                          zero);
-      const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
-      const bool sp = !ONE && __builtin_amdgcn_readlane(int(b1), who) != 0;
-      zero_pivot = zero_pivot || zero;
-      myperm = lane == c ? P + 1 : myperm;
-      // the pivot row moves to position c, the row at c to the pivot's old place
-      pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
-      if constexpr (!ONE) pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
-      const TW pv = wave::rl(sp ? x1 : x0, who);
-      const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
-      // rows below the pivot get the multiplier, the others keep column c
-      const bool u0 = lane < n2 ? pos0 > c : lane < R;
-      const bool u1 = !ONE && lane + 64 < n2 ? pos1 > c : lane + 64 < R;
-      const TW m0 = x0 * rinv, m1 = x1 * rinv;
-      const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
-      V[0][b][cc] = u0 ? m0 : x0;
-      V[1][b][cc] = u1 ? m1 : x1;
-      TW lv2 = TW(0);
-      if constexpr (RS > 2) {   // border rows are never pivots: always updated
-        const bool u2 = lane + 128 < R;
-        const TW m2 = x2 * rinv;
-        lv2 = u2 ? m2 : TW(0);
-        V[RS - 1][b][cc] = u2 ? m2 : x2;
-      }
-      // columns 1..15 of the block (column 0 is never after c)
-      update_after<1, RS, TW>(V[0][b], V[1][b], V[RS - 1][b], sp, who, cc, lv0, lv1, lv2);
-#pragma unroll
-      for (int bb = b + 1; bb < NB; ++bb)
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          const TW u = wave::rl(sp ? V[1][bb][k] : V[0][bb][k], who);
-          V[0][bb][k] = fma(-lv0, u, V[0][bb][k]);
-          V[1][bb][k] = fma(-lv1, u, V[1][bb][k]);
-          if constexpr (RS > 2) V[RS - 1][bb][k] = fma(-lv2, u, V[RS - 1][bb][k]);
-        }
+  // Steps in blocks of 16, each in GR groups of GS steps: step c of block b
+  // reads and rewrites column c through a uniform register index
+  // (s_set_gpr_idx) and updates the columns of blocks b.. with the pivot
+  // row's entries; group g of block b updates its block's columns from
+  // GS g + 1 on, those that may not lie after c getting a zero entry.  Each
+  // group's loop stays rolled (one copy of a step per group).  FP32: four
+  // groups (a quarter of the masked in-block updates); FP64: one (the
+  // copies would cost the FP64 panel a wave per SIMD).
+  constexpr int GR = sizeof(TW) == 4 ? 4 : 1, GS = 16 / GR;
+  // step c of block b; the block's columns from KS on are updated, those
+  // before KM masked
+  auto step = [&](const int b, const int c, auto KS, auto KM) {
+    const int cc = c - 16 * b;
+    const TW x0 = V[0][b][cc], x1 = V[1][b][cc];
+    const TW x2 = RS > 2 ? V[RS - 1][b][cc] : TW(0);   // border rows (row set 2)
+    const bool c0 = lane < n2 && pos0 >= c;
+    const bool c1 = !ONE && lane + 64 < n2 && pos1 >= c;
+    unsigned h0, l0, h1, l1;
+    wave::abs_key(x0, h0, l0);
+    wave::abs_key(x1, h1, l1);
+    // the better of this lane's two candidates (ties to the smaller position)
+    const bool b1 = c1 & (!c0 | (h1 > h0) | ((h1 == h0) & ((l1 > l0) | ((l1 == l0) & (pos1 < pos0)))));
+    bool zero;
+    const int who = wave::argmax_pos<(sizeof(TW) > 4)>(c0 | c1, b1 ? h1 : h0, b1 ? l1 : l0, b1 ? pos1 : pos0, lane,
+                                                        zero);
+    const int P = __builtin_amdgcn_readlane(b1 ? pos1 : pos0, who);
+    const bool sp = !ONE && __builtin_amdgcn_readlane(int(b1), who) != 0;
+    zero_pivot = zero_pivot || zero;
+    myperm = lane == c ? P + 1 : myperm;
+    // the pivot row moves to position c, the row at c to the pivot's old place
+    pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
+    if constexpr (!ONE) pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
+    const TW pv = wave::rl(sp ? x1 : x0, who);
+    const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
+    // rows below the pivot get the multiplier, the others keep column c
+    const bool u0 = lane < n2 ? pos0 > c : lane < R;
+    const bool u1 = !ONE && lane + 64 < n2 ? pos1 > c : lane + 64 < R;
+    const TW m0 = x0 * rinv, m1 = x1 * rinv;
+    const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
+    V[0][b][cc] = u0 ? m0 : x0;
+    V[1][b][cc] = u1 ? m1 : x1;
+    TW lv2 = TW(0);
+    if constexpr (RS > 2) {   // border rows are never pivots: always updated
+      const bool u2 = lane + 128 < R;
+      const TW m2 = x2 * rinv;
+      lv2 = u2 ? m2 : TW(0);
+      V[RS - 1][b][cc] = u2 ? m2 : x2;
     }
+    // columns KS .. 15 of the block (those before are never after c)
+    update_after<decltype(KS)::value, decltype(KM)::value, RS, TW>(V[0][b], V[1][b], V[RS - 1][b], sp, who, cc, lv0,
+                                                                   lv1, lv2);
+#pragma unroll
+    for (int bb = b + 1; bb < NB; ++bb)
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const TW u = wave::rl(sp ? V[1][bb][k] : V[0][bb][k], who);
+        V[0][bb][k] = fma(-lv0, u, V[0][bb][k]);
+        V[1][bb][k] = fma(-lv1, u, V[1][bb][k]);
+        if constexpr (RS > 2) V[RS - 1][bb][k] = fma(-lv2, u, V[RS - 1][bb][k]);
+      }
+  };
+  if constexpr (GR == 1) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int cend = min(16 * b + 16, n);
+      for (int c = 16 * b; c < cend; ++c) step(b, c, std::integral_constant<int, 1>{}, std::integral_constant<int, 16>{});
+    }
+  } else {
+    static_for<0, GR * NB>([&](auto BG) {
+      constexpr int b = decltype(BG)::value / GR, g = decltype(BG)::value % GR;
+      const int cend = min(16 * b + GS * g + GS, n);
+      for (int c = 16 * b + GS * g; c < cend; ++c)
+        step(b, c, std::integral_constant<int, GS * g + 1>{}, std::integral_constant<int, GS * g + GS>{});
+    });
   }
   if (zero_pivot && lane == 0) atomicOr(a.status, 1);
   // T (2n x n, final row order) and C_j U^-1 to their blocks

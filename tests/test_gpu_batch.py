@@ -84,6 +84,34 @@ def test_batch_fp64_matches_oracle_per_system(oracle_mod, kernel):
         assert res <= 1e-12 and abs(res - rep["rel_residual"][s]) <= 0.1 * res + 1e-14, (s, res)
 
 
+@pytest.mark.parametrize("rows", [
+    [(5, 4, 1, 1, 1, 1, 0), (37, 9, 2, 2, 2, 2, 0), (16, 16, 4, 4, 3, 3, 0)],      # last blocks Nc <= 64
+    [(9, 64, 16, 16, 8, 8, 0), (20, 61, 15, 15, 3, 3, 0), (3, 8, 2, 2, 1, 1, 0)],  # Nc up to 152
+], ids=["nc64", "nc152"])
+def test_batch_last_block_solve_widths(oracle_mod, rows):
+    """The one-wavefront last-block solve holds one, two or four rows per
+    lane by the batch's largest last block: FP64 batches whose last blocks
+    need one and four rows per lane against the oracle per system."""
+    dims = np.asarray(rows, np.int32)
+    print(f"case batch fp64 last-block widths Ncmax={max(2 * r[1] + r[2] + r[4] for r in rows)}", flush=True)
+    bt = Batch(mixed=False)
+    bt.plan(dims)
+    bt.generate(seed=11, shift=4.0)
+    b = random_rhs(bt, 3)
+    B = DeviceArray.from_host(b)
+    X = DeviceArray(bt.size("x"))
+    rc = bt.solve(B, X, tol=1e-12, max_iter=0, graph=False)
+    assert rc == 0, bt.report()
+    x = X.to_host()
+    for s, row in enumerate(dims):
+        sysd = bt.system(s)
+        nrows, ncols = systems.dims(sysd)
+        o = bt.offset("x", s)
+        xo = oracle_x(oracle_mod, sysd, int(row[6]), b[o:o + nrows])
+        err = np.abs(x[o:o + ncols] - xo[:ncols]).max() / max(1.0, np.abs(xo[:ncols]).max())
+        assert err < 1e-10, (s, tuple(row), err)
+
+
 @pytest.mark.parametrize("variant", [0, 4, 5])
 def test_batch_mixed_reaches_tolerance_and_graph_is_exact(variant):
     """Mixed batch to 1e-12 with each choice of kernels for the 16 < n <= 48
