@@ -389,10 +389,155 @@ hipError_t launch_assemble_last_batched(const AssembleArgs<TW, TA>* args, int ns
   return hipGetLastError();
 }
 
+// The same factorization (Nc <= 64 RPL) by one workgroup of NWV wavefronts
+// per system with the block in registers: wavefront w holds columns
+// [CW w, CW w + CW) of every row (lane l: rows l + 64 k).  Rows stay in
+// their lanes with their current positions tracked (the pivot row takes
+// position c, the row there the pivot's old place); the wavefront owning
+// column c searches it (largest |a|, ties to the smaller position, as
+// lu_last_body's scan) and publishes the multipliers through LDS, and every
+// wavefront applies the rank-1 update to its columns after c.  The rows are
+// stored back in their final positions with the 1-based ipiv, so the result
+// and its arithmetic per element are lu_last_body's.
+template <int RPL, int NWV>
+__global__ void __launch_bounds__(64 * NWV) lu_last_regs_batched(const LastSolveArgs<double>* args,
+                                                                 int32_t* const* status) {
+  using V16d = double __attribute__((ext_vector_type(16)));
+  constexpr int CW = 64 * RPL / NWV, CB = CW / 16;   // columns and 16-column blocks per wavefront
+  static_assert(CW * NWV == 64 * RPL && CW % 16 == 0, "the waves cover the columns in 16-column blocks");
+  __shared__ double lbuf[2][64 * RPL];
+  __shared__ int pbuf[2][2];
+  const LastSolveArgs<double>& a = args[blockIdx.x];
+  const int Nc = a.Nc;
+  double* const Hm = const_cast<double*>(a.Hm);
+  int32_t* const ipiv = const_cast<int32_t*>(a.P);
+  int32_t* const st = status[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int w = NWV == 1 ? 0 : __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  V16d V[RPL][CB];
+  int pos[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int r = lane + 64 * k;
+    pos[k] = r;
+#pragma unroll
+    for (int jj = 0; jj < CW; ++jj) {
+      const int gj = CW * w + jj;
+      V[k][jj / 16][jj % 16] = r < Nc && gj < Nc ? Hm[r + size_t(gj) * Nc] : 0.0;
+    }
+  }
+  // column blocks of 16 (unrolled: the owning wavefront and its block are
+  // compile-time per block), steps inside a block rolled (the column read
+  // through a uniform register index)
+#pragma unroll
+  for (int gb = 0; gb < 4 * RPL; ++gb) {
+    const int o = 16 * gb / CW, lb = (16 * gb % CW) / 16;
+    const int cend = min(16 * gb + 16, Nc);
+    for (int c = 16 * gb; c < cend; ++c) {
+      const int cc = c - 16 * gb;
+      double lk[RPL];
+      int who = 0, ps = 0, P = 0;
+      if (w == o) {
+        double x[RPL];
+        bool cb = false;
+        unsigned hb = 0, lbits = 0;
+        int pb = 0, kb = 0;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          x[k] = V[k][lb][cc];
+          unsigned h, l;
+          wave::abs_key(x[k], h, l);
+          if (x[k] != x[k]) h = l = 0;   // a NaN is never preferred
+          const bool cand = pos[k] >= c && lane + 64 * k < Nc;
+          const bool better = cand && (!cb || h > hb || (h == hb && (l > lbits || (l == lbits && pos[k] < pb))));
+          if (better) {
+            cb = true;
+            hb = h;
+            lbits = l;
+            pb = pos[k];
+            kb = k;
+          }
+        }
+        bool zero;
+        who = wave::argmax_pos<true>(cb, hb, lbits, pb, lane, zero);
+        P = __builtin_amdgcn_readlane(pb, who);
+        ps = __builtin_amdgcn_readlane(kb, who);
+        double xs = x[0];
+#pragma unroll
+        for (int k = 1; k < RPL; ++k) xs = kb == k ? x[k] : xs;
+        const double pv = wave::rl(xs, who);
+        if (lane == 0) {
+          if (zero) atomicOr(st, 2);
+          ipiv[c] = P + 1;
+        }
+        const double rinv = pv != 0.0 ? 1.0 / pv : 0.0;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const int np = pos[k] == c ? P : pos[k] == P ? c : pos[k];
+          const bool below = np > c && lane + 64 * k < Nc;
+          const double m = x[k] * rinv;
+          lk[k] = below ? m : 0.0;
+          V[k][lb][cc] = below ? m : x[k];
+          if constexpr (NWV > 1) lbuf[c & 1][lane + 64 * k] = lk[k];
+        }
+        if (NWV > 1 && lane == 0) {
+          pbuf[c & 1][0] = who | (ps << 8);
+          pbuf[c & 1][1] = P;
+        }
+      }
+      if constexpr (NWV > 1) {
+        __syncthreads();
+        if (w != o) {
+          const int q = __builtin_amdgcn_readfirstlane(pbuf[c & 1][0]);
+          who = q & 255;
+          ps = q >> 8;
+          P = __builtin_amdgcn_readfirstlane(pbuf[c & 1][1]);
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) lk[k] = lbuf[c & 1][lane + 64 * k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) pos[k] = pos[k] == c ? P : pos[k] == P ? c : pos[k];
+      // this wavefront's columns after c: A[r][j] -= l_r * A[c][j]
+      if (CW * w + CW - 1 > c) {
+#pragma unroll
+        for (int jj = 0; jj < CW; ++jj) {
+          if (CW * w + jj > c) {
+            double v = V[0][jj / 16][jj % 16];
+#pragma unroll
+            for (int k = 1; k < RPL; ++k) v = ps == k ? V[k][jj / 16][jj % 16] : v;
+            const double u = wave::rl(v, who);
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) V[k][jj / 16][jj % 16] -= lk[k] * u;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    if (lane + 64 * k < Nc) {
+#pragma unroll
+      for (int jj = 0; jj < CW; ++jj) {
+        const int gj = CW * w + jj;
+        if (gj < Nc) Hm[pos[k] + size_t(gj) * Nc] = V[k][jj / 16][jj % 16];
+      }
+    }
+  }
+}
+
 // args[s].Hm / .P (ipiv) / .Nc describe the square last block of system s;
 // Ncmax is the largest of the launch
 hipError_t launch_lu_last_batched(const LastSolveArgs<double>* args, int32_t* const* status, int nsys, int Ncmax,
                                   size_t lds_limit, hipStream_t st) {
+  if (Ncmax <= 64) {
+    hipLaunchKernelGGL((lu_last_regs_batched<1, 1>), dim3(nsys), dim3(64), 0, st, args, status);
+    return hipGetLastError();
+  }
+  if (Ncmax <= 128) {
+    hipLaunchKernelGGL((lu_last_regs_batched<2, 4>), dim3(nsys), dim3(256), 0, st, args, status);
+    return hipGetLastError();
+  }
   const size_t bytes = size_t(Ncmax) * Ncmax * sizeof(double);
   if (bytes <= lds_limit)
     hipLaunchKernelGGL((lu_last_batched<true>), dim3(nsys), dim3(1024), bytes, st, args, status);
