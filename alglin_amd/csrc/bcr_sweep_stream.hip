@@ -278,6 +278,93 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
   }
 }
 
+// Back substitution for n <= 32 with two pairs per wavefront: lanes
+// 32 h + r (r < n) own row r of pair 2 w + h, so no lane idles on the upper
+// half; a vector entry of column c is broadcast per half (v_readlane of
+// lanes c and 32 + c, then a select).  One right-hand side, LU exchanges
+// only, pairs in pure cyclic-reduction order (j grows with the pair index:
+// the wavefront's second pair is addressed from its first one's blocks).
+template <typename TW, typename TX>
+__device__ __forceinline__ void backward_lu_stream2_body(SolveArgs<TW, TX> a, int bid) {
+  const int lane = threadIdx.x & 63, h = lane >> 5, r = lane & 31;
+  const int p0 = (bid * 2 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6))) * 2;   // wave-uniform
+  if (p0 >= a.npairs) return;
+  const int p = p0 + h;
+  const bool live = p < a.npairs;
+  const int n = a.n, n2 = 2 * n, nx = a.nx;
+  int j0, jp0, jpp0, j, jp, jpp;
+  pair_index(p0, a.L, a.N, j0, jp0, jpp0);
+  pair_index(live ? p : p0, a.L, a.N, j, jp, jpp);
+  const unsigned dj = unsigned(j - j0);
+  const size_t nn = size_t(n) * n;
+  const TW* Dw = a.D + size_t(j0) * nn;
+  const TW* Ew = a.E + size_t(j0) * nn;
+  const TW* Bw = a.B + size_t(j0) * n * nx;
+  const TW* Tw = a.Tm + size_t(j0) * tsize(n);
+  const bool row = live && r < n;
+  const int rc = row ? r : 0;
+  const unsigned od = dj * unsigned(nn) + unsigned(rc), ob = dj * unsigned(n * nx) + unsigned(rc);
+  const unsigned ot = dj * unsigned(tsize(n)) + unsigned(rc);
+  auto bc = [&](TX v, int c) -> TX {
+    const TX v0 = TX(rlane(double(v), c)), v1 = TX(rlane(double(v), 32 + c));
+    return h ? v1 : v0;
+  };
+  const TX dinv = row ? TX(1) / TX(gload(Tw, ot + unsigned(rc) * n2)) : TX(1);
+  const size_t xbrow = size_t(a.N + 1) * n + a.qx;   // cc:1812
+  TX* const x = a.x;
+  const TX vjp = row ? x[size_t(jp) * n + r] : TX(0);
+  const TX vjpp = row ? x[size_t(jpp) * n + r] : TX(0);
+  TX s = row ? x[size_t(j) * n + r] : TX(0);
+  // D_j x_jp + E_j x_jpp, kChunk columns per memory latency
+  for (int c0 = 0; c0 < n; c0 += kChunk) {
+    TW d[kChunk], e[kChunk];
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n;
+      d[i] = gload(Dw, od + cc);
+      e[i] = gload(Ew, od + cc);
+    }
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const int c = c0 + i;
+      if (c < n) s -= TX(d[i]) * bc(vjp, c) + TX(e[i]) * bc(vjpp, c);
+    }
+  }
+  // B_j xb, eight columns' loads in flight at a time
+  for (int c0 = 0; c0 < nx; c0 += 8) {
+    TW bv[8];
+    TX xv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int cc = c0 + i < nx ? c0 + i : 0;
+      bv[i] = gload(Bw, ob + unsigned(cc) * n);
+      xv[i] = x[xbrow + cc];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (c0 + i < nx) s -= TX(keep_if(bv[i], row)) * xv[i];
+  }
+  // U^{-1}, bottom up: lane 32 h + c owns the diagonal of column c
+  for (int c1 = n - 1; c1 >= 0; c1 -= kChunk) {
+    TW u[kChunk];
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const int c = c1 - i;
+      u[i] = gload(Tw, ot + unsigned(c >= 0 ? c : 0) * n2);
+    }
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const int c = c1 - i;
+      if (c >= 0) {
+        const TX xc = bc(s * dinv, c);
+        if (r < c) s -= TX(u[i]) * xc;
+        else if (r == c) s = xc;
+      }
+    }
+  }
+  if (row) x[size_t(j) * n + r] = s;
+}
+
 template <typename TW, typename TX, bool FWD, int QB>
 __global__ void __launch_bounds__(256) sweep_lu_stream_kernel(SolveArgs<TW, TX> a) {
   if (FWD) forward_lu_stream_body<TW, TX, QB>(a, blockIdx.x);
@@ -290,6 +377,14 @@ __global__ void __launch_bounds__(256) sweep_lu_stream_batched(BatchTable<SolveA
   if (skipped(t.args[e].skip)) return;
   if (FWD) forward_lu_stream_body<TW, TX, 1>(t.args[e], int(blockIdx.x) - t.first[e]);
   else backward_lu_stream_body<TW, TX, 1>(t.args[e], int(blockIdx.x) - t.first[e]);
+}
+
+// four pairs per workgroup of two wavefronts (the batch tables' grouping)
+template <typename TW, typename TX>
+__global__ void __launch_bounds__(128) sweep_lu_stream2_back_batched(BatchTable<SolveArgs<TW, TX>> t) {
+  const int e = t.entry[blockIdx.x];
+  if (skipped(t.args[e].skip)) return;
+  backward_lu_stream2_body<TW, TX>(t.args[e], int(blockIdx.x) - t.first[e]);
 }
 
 }  // namespace
@@ -314,7 +409,9 @@ template <typename TW, typename TX>
 bool launch_sweep_lu_stream_batched(bool forward, BatchTable<SolveArgs<TW, TX>> const& t, int nblocks, int n,
                                     hipStream_t st, hipError_t* err) {
   if (n < 1 || n > 64) return false;
+  // (the batch tables hold pure cyclic-reduction pairs, four per workgroup)
   if (forward) hipLaunchKernelGGL((sweep_lu_stream_batched<TW, TX, true>), dim3(nblocks), dim3(256), 0, st, t);
+  else if (n <= 32) hipLaunchKernelGGL((sweep_lu_stream2_back_batched<TW, TX>), dim3(nblocks), dim3(128), 0, st, t);
   else hipLaunchKernelGGL((sweep_lu_stream_batched<TW, TX, false>), dim3(nblocks), dim3(256), 0, st, t);
   *err = hipGetLastError();
   return true;
