@@ -14,10 +14,12 @@
 //     arithmetic of the other LU kernels, so T, the pivots and C_j U^-1
 //     agree with them bit for bit.  The step loop is rolled within each
 //     16-column block (column c read and written through a uniform register
-//     index), a few KB of code.  Rows do not move; each row is stored at its
-//     final position (T, C_j U^-1, perm).
+//     index; FP32: in groups of four steps, each updating its block's
+//     columns from the group's first on).  Rows do not move; each row is
+//     stored at its final position (T, C_j U^-1, perm), and the positions
+//     themselves go to the unused tail of T (one byte per row).
 //  B. pair_rhs_mfma: the right-hand part X = [col jp | col jpp | xb], two
-//     waves per pair.  The row positions are replayed from perm; LDS gets
+//     waves per pair.  The row positions come from A's bytes; LDS gets
 //     -L11 below its 16 x 16 diagonal blocks, the inverses of those unit
 //     lower blocks on them (forward substitution, one column per lane), and
 //     -[L21; C_j U^-1].  X goes in 16-column tiles, each inside one of the
@@ -345,18 +347,18 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
     const TW pv = wave::rl(sp ? x1 : x0, who);
     const TW rinv = pv != TW(0) ? TW(1) / pv : TW(0);
     // rows below the pivot get the multiplier, the others keep column c
-    const bool u0 = lane < n2 ? pos0 > c : lane < R;
-    const bool u1 = !ONE && lane + 64 < n2 ? pos1 > c : lane + 64 < R;
+    // (border and padding rows hold positions r >= 2n > c: always below;
+    // the padding rows are zero and never stored)
+    const bool u0 = pos0 > c;
+    const bool u1 = pos1 > c;
     const TW m0 = x0 * rinv, m1 = x1 * rinv;
     const TW lv0 = u0 ? m0 : TW(0), lv1 = u1 ? m1 : TW(0);
     V[0][b][cc] = u0 ? m0 : x0;
     V[1][b][cc] = u1 ? m1 : x1;
     TW lv2 = TW(0);
-    if constexpr (RS > 2) {   // border rows are never pivots: always updated
-      const bool u2 = lane + 128 < R;
-      const TW m2 = x2 * rinv;
-      lv2 = u2 ? m2 : TW(0);
-      V[RS - 1][b][cc] = u2 ? m2 : x2;
+    if constexpr (RS > 2) {   // border (and zero padding) rows: never pivots, always updated
+      lv2 = x2 * rinv;
+      V[RS - 1][b][cc] = lv2;
     }
     // columns KS .. 15 of the block (those before are never after c)
     update_after<decltype(KS)::value, decltype(KM)::value, RS, TW>(V[0][b], V[1][b], V[RS - 1][b], sp, who, cc, lv0,
@@ -403,6 +405,12 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
     }
   }
   if (lane < n) a.perm[size_t(j) * n + lane] = myperm;
+  // the final position of every panel row, one byte each after the panel's
+  // 2n^2 entries (the LU pair leaves that tail of T unused): kernel B reads
+  // them instead of replaying the swap sequence
+  uint8_t* const posb = reinterpret_cast<uint8_t*>(Tj + size_t(2) * n * n);
+  if (lane < n2) posb[lane] = uint8_t(pos0);
+  if (lane + 64 < n2) posb[lane + 64] = uint8_t(pos1);
 }
 
 // ---- B: the right-hand part on the matrix cores ----------------------------------
@@ -428,7 +436,10 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
   // again, a dependent memory round trip in front of the load it addresses)
   const TS *const sD = a.sD, *const sE = a.sE, *const sB = a.sB, *const sC = a.sC;
   TW *const Dm = a.D, *const Em = a.E, *const Bm = a.B, *const Cm = a.C, *const Fm = a.Fpart;
-  const int32_t* const perm = a.perm;
+  // the final positions of this lane's two panel rows, stored by kernel A
+  // after the panel's 2n^2 entries
+  const uint8_t* const posb = reinterpret_cast<const uint8_t*>(Tj + size_t(2) * n * n);
+  const uint8_t pb0 = posb[lane < n2 ? lane : 0], pb1 = posb[lane + 64 < n2 ? lane + 64 : 0];
 
   // this lane's two panel rows as right-hand sources, and the first tile's
   // loads, in flight while the multipliers are staged
@@ -506,16 +517,8 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
       for (int i = n + nr; i < NR; ++i) Ls[(NT + i) * LD + k] = TW(0);
     }
   }
-  // the final position of this lane's two rows: LAPACK's swap sequence replayed
-  int pos0 = lane, pos1 = lane + 64;
-  {
-    const int pv = lane < n ? perm[size_t(j) * n + lane] - 1 : 0;
-    for (int c = 0; c < n; ++c) {
-      const int P = __builtin_amdgcn_readlane(pv, c);
-      pos0 = pos0 == c ? P : pos0 == P ? c : pos0;
-      pos1 = pos1 == c ? P : pos1 == P ? c : pos1;
-    }
-  }
+  // the final position of this lane's two rows (kernel A's bytes)
+  const int pos0 = lane < n2 ? int(pb0) : lane, pos1 = lane + 64 < n2 ? int(pb1) : lane + 64;
   __syncthreads();   // Ls loaded
   if (lane < NT && (lane >> 4) % NW == w) {
     // the inverse of diagonal block lane / 16 of L11 (unit lower), column
