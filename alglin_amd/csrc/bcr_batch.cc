@@ -136,7 +136,8 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
   dims_ = dims;
   const int ns = nsys();
   for (auto& v : off_) v.assign(ns + 1, 0);
-  for (auto* v : {&offT_, &offPerm_, &offCpend_, &offFpart_, &offHm_, &offPiv_, &offXpart_, &offCpart_, &offCsum_})
+  for (auto* v : {&offT_, &offPerm_, &offCpend_, &offFpart_, &offHm_, &offPiv_, &offXpart_, &offCpart_, &offCsum_,
+                  &offPst_})
     v->assign(ns + 1, 0);
   maxN_ = Ncmax_ = Nrmax_ = 0;
   square_ = true;
@@ -159,6 +160,7 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
     offXpart_[s + 1] = offXpart_[s] + N * nr;
     offCpart_[s + 1] = offCpart_[s] + (N + 1) * nr;
     offCsum_[s + 1] = offCsum_[s] + nr;
+    offPst_[s + 1] = offPst_[s] + 3 * std::max<int64_t>(N - 1, 0);
     maxN_ = std::max(maxN_, d.N);
     Ncmax_ = std::max(Ncmax_, d.Nc());
     Nrmax_ = std::max(Nrmax_, d.Nr());
@@ -183,6 +185,8 @@ void BatchEngine<TA, TW>::plan(std::vector<SysDims> const& dims) {
   Xpart_.resize(one(offXpart_[ns]));
   Cpart_.resize(one(offCpart_[ns]));
   Csum_.resize(one(offCsum_[ns]));
+  pstat_.resize(one(offPst_[ns]));
+  stats_ok_ = false;
   factor_bytes_ = sizeof(TW) * (2 * tot_[kbDE] + tot_[kbB] + tot_[kbC] + offT_[ns] + offCpend_[ns] + offFpart_[ns]) +
                   4 * offPerm_[ns] + 8 * offHm_[ns];
   active_.resize(ns);
@@ -367,13 +371,16 @@ void BatchEngine<TA, TW>::build_factor_tables() {
         a.Tm = Tm_.p + offT_[s]; a.perm = perm_.p + offPerm_[s];
         a.Cpend = Cpend_.p + offCpend_[s]; a.Fpart = Fpart_.p + offFpart_[s];
         a.status = status_.p + s;
+        // FP32 LU panels: the selector's per-panel statistics, formed by the
+        // pair kernels where they hold the panel anyway
+        a.pstat = sizeof(TW) == 4 && cls <= 4 ? pstat_.p + offPst_[s] : nullptr;
         if (L == 1) {
           FactorArgs<TA, TW> b{};
           b.N = a.N; b.n = a.n; b.nr = a.nr; b.nx = a.nx; b.L = a.L; b.npairs = a.npairs;
           b.slot_cur = a.slot_cur; b.first = 1;
           b.sD = pD_ + off_[kbDE][s]; b.sE = pE_ + off_[kbDE][s]; b.sB = pB_ + off_[kbB][s]; b.sC = pC_ + off_[kbC][s];
           b.D = a.D; b.E = a.E; b.B = a.B; b.C = a.C; b.Tm = a.Tm; b.perm = a.perm; b.Cpend = a.Cpend;
-          b.Fpart = a.Fpart; b.status = a.status;
+          b.Fpart = a.Fpart; b.status = a.status; b.pstat = a.pstat;
           a0.push_back(b);
         } else {
           a.sD = a.D; a.sE = a.E; a.sB = a.B; a.sC = a.C;
@@ -462,6 +469,9 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
   const bool conc = ev == nullptr && streams_ > 1 && fork_level_ < levels_;
   bool forked = false;
   size_t li = 0;
+  // the launches' kernels all write the panels' statistics (FP32 LU: the
+  // register kernels n <= 16 and the matrix-core pair)
+  bool stats = sizeof(TW) == 4;
   for (int k = 0; k < levels_; ++k)
     for (Launch const& l : fac_[k]) {
       if (conc && !forked && k >= fork_level_) {
@@ -479,10 +489,13 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
         // default of pair_variant_
         const int ck = cls_kernel_[l.cls];
         const int v = ck == kKernMfma ? 5 : ck >= 0 ? 4 : pair_variant_;
-        if (l.cls <= 2) ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
-        else if ((l.cls == 3 || l.cls == 4) && try_mfma(v, t, l, s, &e))
-          ok = true;
-        else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, s, &e);
+        bool wrote = false;   // the kernel writes the panels' statistics
+        if (l.cls <= 2) {
+          ok = launch_pair_lu_fast_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e);
+          wrote = true;
+        } else if ((l.cls == 3 || l.cls == 4) && try_mfma(v, t, l, s, &e)) {
+          wrote = true;
+        } else if (l.cls == 3) ok = launch_pair_lu_split_batched(t, l.nblocks, 32, l.nxmax, l.nrmax, s, &e);
         else if (l.cls == 4 && ck != kKernTiled && pair_lu_cols_fits(l.nmax, l.nrmax, l.nxmax))
           ok = launch_pair_lu_cols_batched(t, l.nblocks, l.nmax, l.nxmax, l.nrmax, s, &e, cls_waves_[l.cls]);
         else if (l.cls == 4 && pair_lu_tiled_fits(l.nmax, l.nrmax, l.nxmax))
@@ -495,6 +508,7 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
           e = launch_pair_generic_batched(l.method, t, l.nblocks, bytes, lds_limit_, s);
         }
         if (!ok) throw HipError("batch factorize: no kernel for a class");
+        stats = stats && wrote;
       };
       if (k == 0) run(static_cast<const FactorArgs<TA, TW>*>(l.args));
       else run(static_cast<const FactorArgs<TW, TW>*>(l.args));
@@ -503,6 +517,7 @@ void BatchEngine<TA, TW>::factorize_impl(std::vector<hipEvent_t>* ev) {
       ++li;
     }
   if (forked) join();
+  stats_ok_ = stats;
   hip_check(launch_convert<TA, double>(pF_, Feff_.p, tot_[kbF], st_), "F");
   hip_check(launch_reduce_batched<TW, double>(fred_, ns, st_), "reduce F");
   hip_check(launch_assemble_last_batched<TW, TA>(asm_, ns, st_), "assemble last");
@@ -820,21 +835,16 @@ void BatchEngine<TA, TW>::features_async(double const* norms, float* out) {
     // panels j = 1 .. N-1 of the active systems, in order, four per workgroup
     std::vector<FeatArgs<TW>> f(active_.size());
     std::vector<int32_t> entry, first(active_.size());
-    std::vector<size_t> pan(active_.size());
-    size_t np = 0;
     for (size_t i = 0; i < active_.size(); ++i) {
       const int s = active_[i];
       SysDims const& d = dims_[s];
       FeatArgs<TW>& a = f[i];
       a.N = d.N; a.n = d.n; a.qr = d.qr; a.nr = d.nr; a.Nc = d.Nc(); a.sys = s;
       a.Tm = Tm_.p + offT_[s]; a.Hm = Hm_.p + offHm_[s]; a.norms = norms;
-      pan[i] = np;
-      np += size_t(std::max(d.N - 1, 0));
+      a.pstat = pstat_.p + offPst_[s];
       first[i] = int32_t(entry.size());
       for (int k = 0; k < (d.N - 1 + 3) / 4; ++k) entry.push_back(int32_t(i));
     }
-    pstat_.resize(3 * std::max<size_t>(np, 1));
-    for (size_t i = 0; i < f.size(); ++i) f[i].pstat = pstat_.p + 3 * pan[i];
     Blob bb;
     const size_t oa = bb.add(f), oe = bb.add(entry), of = bb.add(first);
     bb.upload(feat_blob_, st_);
@@ -846,7 +856,9 @@ void BatchEngine<TA, TW>::features_async(double const* norms, float* out) {
     feat_norms_ = norms;
     feat_ok_ = true;
   }
-  hip_check(launch_panel_stats_batched<TW>(feat_tab_, feat_wgs_, st_), "panel statistics");
+  // the pair kernels of the last factorization formed the statistics where
+  // they held the panels; else a pass over the T panels does
+  if (!stats_ok_) hip_check(launch_panel_stats_batched<TW>(feat_tab_, feat_wgs_, st_), "panel statistics");
   hip_check(launch_features_batched<TW>(feat_args_, feat_nsys_, out, st_), "features");
 }
 

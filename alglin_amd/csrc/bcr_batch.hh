@@ -190,7 +190,7 @@ class BatchEngine {
   std::vector<int64_t> off_[kbKinds];
   int64_t tot_[kbKinds] = {0, 0, 0, 0, 0, 0, 0};
   // factor-side offsets
-  std::vector<int64_t> offT_, offPerm_, offCpend_, offFpart_, offHm_, offPiv_, offXpart_, offCpart_, offCsum_;
+  std::vector<int64_t> offT_, offPerm_, offCpend_, offFpart_, offHm_, offPiv_, offXpart_, offCpart_, offCsum_, offPst_;
   int maxN_ = 0, levels_ = 0, Ncmax_ = 0, Nrmax_ = 0;
   bool square_ = true;   // every last block square (Nr == Nc)
   size_t lds_limit_ = 65536, factor_bytes_ = 0;
@@ -230,7 +230,10 @@ class BatchEngine {
   // the feature launches' records and tables (active set and norms pointer
   // of the call that built them), and the per-panel statistics
   DevBuf<char> feat_blob_;
-  DevBuf<float> pstat_;
+  DevBuf<float> pstat_;   // 3 floats per panel j = 1 .. N-1 of every system (offPst_)
+  // the last factorization left every active panel's statistics in pstat_
+  // (FP32 LU classes on the kernels that write them)
+  bool stats_ok_ = false;
   const FeatArgs<TW>* feat_args_ = nullptr;
   BatchTable<FeatArgs<TW>> feat_tab_{};
   int feat_wgs_ = 0, feat_nsys_ = 0;

@@ -45,6 +45,10 @@ struct FactorArgs {
   // partitioned elimination order: (jp, j, jpp) of pair p at pairs[3p..3p+2];
   // nullptr = pure cyclic reduction over [0, N) at stride L
   const int32_t* pairs = nullptr;
+  // FP32 LU pairs of a batch: the selector's statistics of panel j (min and
+  // max |U_cc|, max |U_rc| over r <= c of its top n x n block) at
+  // pstat[3 (j - 1) ..], written by the pair kernels that can; nullptr = none
+  float* pstat = nullptr;
 };
 
 // Per-level arguments of the forward / backward sweeps.

@@ -420,6 +420,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
   constexpr int NT = G::NT, NR = G::NR, LD = G::LD, XLD = G::XLD;
   __shared__ __align__(16) TW Ls[(NT + NR) * LD];
   __shared__ __align__(16) TW Xt[NW][16 * XLD];
+  __shared__ unsigned sstat[NW][3];   // per-wave panel statistics
   if (p >= a.npairs) return;   // uniform over the workgroup
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -487,6 +488,11 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
 #pragma unroll
       for (int ri = 0; ri < kNrB; ++ri) cv[ri] = Cm[j * rn + (ri < nr ? ri : 0) + size_t(kk) * nr];
     }
+    // the selector's statistics of this panel (FP32 batches): |U_qq| min and
+    // max, |U_qk| max over q <= k, as float bits (non-negative: ordered as
+    // unsigned), from the U entries this wave reads anyway
+    const bool stats = sizeof(TW) == 4 && a.pstat != nullptr;
+    unsigned dmin = 0xffffffffu, dmax = 0u, umax = 0u;
 #pragma unroll
     for (int h = 0; h < PAIRS; ++h) {
       const int q0 = 2 * (w + NW * h);
@@ -500,12 +506,30 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
           if (q < n) {
             row = q;
             x = k < q ? ((k >> 4) == (q >> 4) ? v : -v) : TW(0);
+            if (stats && k >= q && k < n) {
+              const unsigned bits = __float_as_uint(fabsf(float(v)));
+              umax = max(umax, bits);
+              if (k == q) {
+                dmin = min(dmin, bits);
+                dmax = max(dmax, bits);
+              }
+            }
           } else {
             row = NT + q - n;
             x = -v;
           }
           Ls[row * LD + k] = keep_if(x, k < n);
         }
+      }
+    }
+    if (stats) {
+      dmin = wave::min_u32(dmin);
+      dmax = wave::max_u32(dmax);
+      umax = wave::max_u32(umax);
+      if (lane == 0) {
+        sstat[w][0] = dmin;
+        sstat[w][1] = dmax;
+        sstat[w][2] = umax;
       }
     }
     // border rows (C_j U^-1, nr x n column-major) and the zero padding rows
@@ -520,6 +544,19 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
   // the final position of this lane's two rows (kernel A's bytes)
   const int pos0 = lane < n2 ? int(pb0) : lane, pos1 = lane + 64 < n2 ? int(pb1) : lane + 64;
   __syncthreads();   // Ls loaded
+  if (sizeof(TW) == 4 && a.pstat != nullptr && w == 0 && lane == 0) {
+    unsigned dmin = sstat[0][0], dmax = sstat[0][1], umax = sstat[0][2];
+#pragma unroll
+    for (int v = 1; v < NW; ++v) {
+      dmin = min(dmin, sstat[v][0]);
+      dmax = max(dmax, sstat[v][1]);
+      umax = max(umax, sstat[v][2]);
+    }
+    float* const o = a.pstat + 3 * size_t(j - 1);
+    o[0] = __uint_as_float(dmin);
+    o[1] = __uint_as_float(dmax);
+    o[2] = __uint_as_float(umax);
+  }
   if (lane < NT && (lane >> 4) % NW == w) {
     // the inverse of diagonal block lane / 16 of L11 (unit lower), column
     // lane % 16, by forward substitution in place (the blocks dealt out to
