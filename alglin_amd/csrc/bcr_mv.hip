@@ -244,10 +244,84 @@ __device__ __forceinline__ void mv_rows_multi_run(MvArgs<TA> const& a, int bid) 
   }
 }
 
+// Block row k < N on one wave when n + nr <= 64: its n rows on lanes [0, n)
+// and its nr border partials C_k x_k on lanes [n, n + nr).  The border lanes
+// read C_k through the same load instructions as the row lanes read D_k (one
+// pointer and column stride per lane), so a column costs two loads instead of
+// three; their E_k products are multiplied by zero.  Sums and their order are
+// those of mv_rows_run.  (The same split for the packed layout, n + nr lanes
+// per block row, was no faster there: fewer block rows per wave and more
+// registers ate the saved loads.)
+template <typename TA, bool ABS>
+__device__ __forceinline__ void mv_rows_wide(MvArgs<TA> const& a, int k) {
+  constexpr int CH = 8, CB = 2;
+  const int lane = threadIdx.x & 63;
+  const int n = a.n, nx = a.nx, nr = a.nr;
+  const bool row = lane < n, brow = !row && lane < n + nr;
+  const int rr = row ? lane : 0;
+  const double* xk = a.x + size_t(k) * n;
+  const double* xb = a.x + size_t(a.N + 1) * n + a.qx;
+  const double xa = mag<ABS>(xk[rr]);       // x_k (lane c holds column c for c < n)
+  const double xn = mag<ABS>(xk[n + rr]);   // x_{k+1}
+  const TA* P = brow ? a.C + size_t(k) * nr * n + (lane - n) : a.D + size_t(k) * n * n + rr;
+  const unsigned ps = brow ? unsigned(nr) : unsigned(n);   // column stride of this lane's block
+  const TA* Ek = a.E + size_t(k) * n * n;
+  const TA* Bk = a.B + size_t(k) * n * nx;
+  TA bb0[CB];
+  double w0[CB];
+#pragma unroll
+  for (int i = 0; i < CB; ++i) {
+    const int cc = i < nx ? i : 0;
+    bb0[i] = nx > 0 ? gload(Bk, unsigned(rr + cc * n)) : TA(0);
+    w0[i] = nx > 0 ? mag<ABS>(xb[cc]) : 0.0;
+  }
+  const size_t out = size_t(k) * n + rr;
+  const double bval = a.b ? a.b[out] : 0.0;
+  double acc = 0;
+  for (int c0 = 0; c0 < n; c0 += CH) {
+    TA d[CH], e[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int cc = c0 + i < n ? c0 + i : 0;
+      d[i] = gload(P, unsigned(cc) * ps);
+      e[i] = gload(Ek, unsigned(rr + cc * n));
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int cc = c0 + i < n ? c0 + i : 0;
+      const double u = wave::rl(xa, cc), v = brow ? 0.0 : wave::rl(xn, cc);
+      acc += keep_if(mag<ABS>(double(d[i])) * u + mag<ABS>(double(e[i])) * v, c0 + i < n);
+    }
+  }
+  if (brow) a.Cpart[size_t(k) * nr + (lane - n)] = acc;
+#pragma unroll
+  for (int i = 0; i < CB; ++i) acc += keep_if(mag<ABS>(double(bb0[i])) * w0[i], i < nx);
+  for (int c0 = CB; c0 < nx; c0 += CH) {
+    TA bb[CH];
+    double w[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int cc = c0 + i < nx ? c0 + i : 0;
+      bb[i] = gload(Bk, unsigned(rr + cc * n));
+      w[i] = mag<ABS>(xb[cc]);
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc += keep_if(mag<ABS>(double(bb[i])) * w[i], c0 + i < nx);
+  }
+  if (row) a.out[out] = ABS ? acc + fabs(bval) : (a.b ? bval - acc : acc);
+}
+
 template <typename TA, bool ABS = false>
 __device__ __forceinline__ void mv_rows_body(MvArgs<TA> const& a, int bid) {
-  if (mv_rows_per_wave(a.n, a.nr) > 1) mv_rows_run<TA, true, ABS>(a, bid);
-  else mv_rows_run<TA, false, ABS>(a, bid);
+  if (mv_rows_per_wave(a.n, a.nr) > 1) {
+    mv_rows_run<TA, true, ABS>(a, bid);
+  } else if (a.nr > 0 && a.n + a.nr <= 64) {
+    const int k = __builtin_amdgcn_readfirstlane(bid * 4 + int(threadIdx.x >> 6));
+    if (k < a.N) mv_rows_wide<TA, ABS>(a, k);
+    else mv_rows_run<TA, false, ABS>(a, bid);   // block row N: its border partial (nothing past it)
+  } else {
+    mv_rows_run<TA, false, ABS>(a, bid);
+  }
 }
 
 template <typename TA, bool ABS>
