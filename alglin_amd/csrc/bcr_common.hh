@@ -193,6 +193,21 @@ __device__ __forceinline__ void pair_index(int p, int L, int N, int& j, int& jp,
 // partitioned order of BABD_BorderedCR.cc:1072-1080 and 1181-1190: block
 // ranges, then the chain of their end blocks), else pure cyclic reduction
 template <typename Args>
+__device__ __forceinline__ void pair_of(Args const& a, int p, int& j, int& jp, int& jpp);
+
+// pair_of for a wave-uniform p: the block indices are moved into scalar
+// registers, so addresses built from them keep a scalar base (the table
+// lookup alone would leave them in vector registers, and every load from the
+// pair's blocks would carry a 64-bit vector address)
+template <typename Args>
+__device__ __forceinline__ void pair_of_uniform(Args const& a, int p, int& j, int& jp, int& jpp) {
+  pair_of(a, p, j, jp, jpp);
+  j = __builtin_amdgcn_readfirstlane(j);
+  jp = __builtin_amdgcn_readfirstlane(jp);
+  jpp = __builtin_amdgcn_readfirstlane(jpp);
+}
+
+template <typename Args>
 __device__ __forceinline__ void pair_of(Args const& a, int p, int& j, int& jp, int& jpp) {
   if (a.pairs != nullptr) {
     jp = a.pairs[3 * p];

@@ -141,7 +141,7 @@ __device__ __forceinline__ void pair_lu_cols_body(FactorArgs<TS, TW> const& a, i
   const int n = a.n, nr = a.nr, nx = a.nx, n2 = 2 * n;
   const int R = n2 + nr, no = n2 + nx;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const size_t rn = size_t(nr) * n;
   TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
   TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;

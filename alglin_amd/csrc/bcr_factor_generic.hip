@@ -70,7 +70,7 @@ __device__ __forceinline__ void pair_generic_body(FactorArgs<TS, TW> a, TW* __re
   const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const int n = a.n, nr = a.nr, nx = a.nx;
   const int n2 = 2 * n;
   const int R = n2 + nr;

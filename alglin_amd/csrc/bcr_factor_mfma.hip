@@ -264,7 +264,7 @@ __device__ __forceinline__ void pair_lu_t_body(FactorArgs<TS, TW> const& a, int 
   const int n = a.n, nr = a.nr, n2 = 2 * n;
   const int R = n2 + nr;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const size_t nn = size_t(n) * n, rn = size_t(nr) * n;
   const TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
   const TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;
@@ -427,7 +427,7 @@ __device__ __forceinline__ void pair_rhs_body(FactorArgs<TS, TW> const& a, int p
   const int n = a.n, nr = a.nr, nx = a.nx, n2 = 2 * n;
   const int R = n2 + nr;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const size_t nn = size_t(n) * n, rn = size_t(nr) * n, bn = size_t(n) * nx;
   TW* pend_cur = a.Cpend + size_t(a.slot_cur) * (a.N + 1) * rn;
   TW* pend_prv = a.Cpend + size_t(a.slot_cur ^ 1) * (a.N + 1) * rn;

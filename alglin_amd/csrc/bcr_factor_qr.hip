@@ -141,7 +141,7 @@ __device__ __forceinline__ void pair_qr_body(FactorArgs<TS, TW> const& a, int cw
   __shared__ int s_map[64];
   if (p >= a.npairs) return;   // uniform over the workgroup
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const int n = a.n, nr = a.nr, nx = a.nx, n2 = 2 * n, R = n2 + nr;
   const int ld = R | 1;
   const int Wr = n2 + nx;   // right-hand columns

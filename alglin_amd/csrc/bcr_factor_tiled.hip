@@ -40,7 +40,7 @@ __device__ __forceinline__ void pair_lu_tiled_body(FactorArgs<TS, TW> a, int bid
   const int p = bid;
   if (p >= a.npairs) return;   // uniform over the workgroup
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const int n = a.n, nr = a.nr, nx = a.nx, n2 = 2 * n;
   const int R = n2 + nr, Wc = 3 * n + nx;
   const int tid = threadIdx.x, tr = tid % TR, tc = tid / TR;

@@ -22,7 +22,7 @@ __device__ __forceinline__ void forward_body(SolveArgs<TW, TX> a, int bid) {
   const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const int n = a.n, n2 = 2 * n, nr = a.nr;
   const int tid = threadIdx.x, nth = blockDim.x;
   TW* Tp = reinterpret_cast<TW*>(smem_raw);                                     // 2n*n + n
@@ -114,7 +114,7 @@ __device__ __forceinline__ void backward_body(SolveArgs<TW, TX> a, int bid) {
   const int p = bid;
   if (p >= a.npairs) return;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const int n = a.n, n2 = 2 * n, nx = a.nx;
   const int tid = threadIdx.x, nth = blockDim.x;
   const size_t nn = size_t(n) * n;

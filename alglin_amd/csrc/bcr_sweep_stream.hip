@@ -43,7 +43,7 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
   if (p >= a.npairs) return;   // uniform over the wavefront
   const int n = a.n, n2 = 2 * n, nr = a.nr, nrhs = a.nrhs;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
   const int r0 = lane, r1 = lane + 64;
   const bool v0 = r0 < n2, v1 = r1 < n2;
@@ -117,7 +117,7 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
   if (p >= a.npairs) return;
   const int n = a.n, n2 = 2 * n, nx = a.nx, nrhs = a.nrhs;
   int j, jp, jpp;
-  pair_of(a, p, j, jp, jpp);
+  pair_of_uniform(a, p, j, jp, jpp);
   const size_t nn = size_t(n) * n;
   const TW* Dj = a.D + size_t(j) * nn;
   const TW* Ej = a.E + size_t(j) * nn;
