@@ -53,6 +53,11 @@ __device__ __forceinline__ void forward_lu_fast_body(SolveArgs<TW, TX> a, int bi
   const int64_t ldx = a.ldx;
   int j = 0, jp = 0, jpp = 0;
   if (live) pair_of(a, p, j, jp, jpp);
+  if constexpr (G == 64) {   // one pair per wavefront: scalar block bases
+    j = __builtin_amdgcn_readfirstlane(j);
+    jp = __builtin_amdgcn_readfirstlane(jp);
+    jpp = __builtin_amdgcn_readfirstlane(jpp);
+  }
   const bool row = live && r < n2;
 
   // row r of the 2n x n panel [L; M] (column-major, ld 2n)
@@ -158,6 +163,11 @@ __device__ __forceinline__ void backward_lu_fast_body(SolveArgs<TW, TX> a, int b
   const int64_t ldx = a.ldx;
   int j = 0, jp = 0, jpp = 0;
   if (live) pair_of(a, p, j, jp, jpp);
+  if constexpr (G == 64) {   // one pair per wavefront: scalar block bases
+    j = __builtin_amdgcn_readfirstlane(j);
+    jp = __builtin_amdgcn_readfirstlane(jp);
+    jpp = __builtin_amdgcn_readfirstlane(jpp);
+  }
   const int h = r / NMAX, rr = r % NMAX;
   const bool row = live && rr < n;
   const size_t nn = size_t(n) * n;
