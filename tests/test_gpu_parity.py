@@ -177,12 +177,16 @@ def test_large_n32_matches_oracle(oracle_mod):
     assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-13
 
 
-@pytest.mark.parametrize("n,nr,nx", [(40, 3, 2), (48, 8, 16), (33, 2, 9), (20, 5, 9), (6, 1, 1), (8, 0, 0)])
+@pytest.mark.parametrize("n,nr,nx", [(40, 3, 2), (48, 8, 16), (33, 2, 9), (20, 5, 9), (6, 1, 1), (8, 0, 0),
+                                     (56, 8, 3), (48, 16, 2), (60, 8, 2), (64, 2, 2), (44, 0, 2)])
 def test_device_residual_shapes(n, nr, nx):
     """The GPU residual r = b - A x (BABD_BorderedCR.cc:2111-2166): block rows
-    with the border partials C_k x_k formed in the same column loop, the
-    first B columns and b loaded ahead, several block rows per wave for small
-    n; against the host product of the same blocks."""
+    with the border partials C_k x_k formed in the same column loop (on
+    lanes n..n+nr-1 of their own when n + nr <= 64 and one block row fills a
+    wave: (56, 8) and (48, 16) fill the wave exactly, (60, 8) and (64, 2) do
+    not fit, nr = 0 has no border), the first B columns and b loaded ahead,
+    several block rows per wave for small n; against the host product of the
+    same blocks."""
     from alglin_amd.bcr import DeviceArray, packed_blocks
     N = 37
     qr = max(min(4, n), nx - nr)
