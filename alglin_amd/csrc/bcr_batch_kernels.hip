@@ -61,7 +61,24 @@ __global__ void __launch_bounds__(256) norms_batched(const NormArgs* args, doubl
   const NormArgs& a = args[blockIdx.x];
   if (skipped(a.skip)) return;
   double sb = 0, sr = 0;
-  for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) {
+  // four elements' loads in flight per thread; each thread still sums its
+  // elements i, i + blockDim, i + 2 blockDim, ... in that order
+  const int64_t st = blockDim.x;
+  int64_t i = threadIdx.x;
+  for (; i + 3 * st < a.len; i += 4 * st) {
+    double vb[4], vr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      vb[u] = a.b[i + u * st];
+      vr[u] = a.r[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sb += vb[u] * vb[u];
+      sr += vr[u] * vr[u];
+    }
+  }
+  for (; i < a.len; i += st) {
     const double vb = a.b[i], vr = a.r[i];
     sb += vb * vb;
     sr += vr * vr;
@@ -294,7 +311,19 @@ __global__ void __launch_bounds__(256) converge_batched(ConvergeArgs a) {
 __global__ void __launch_bounds__(256) add_batched(const CopyArgs* args) {
   const CopyArgs& a = args[blockIdx.x];
   if (skipped(a.skip)) return;
-  for (int64_t i = threadIdx.x; i < a.len; i += blockDim.x) a.dst[i] += a.src[i];
+  const int64_t st = blockDim.x;
+  int64_t i = threadIdx.x;
+  for (; i + 3 * st < a.len; i += 4 * st) {   // four elements' loads in flight per thread
+    double d[4], v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      d[u] = a.dst[i + u * st];
+      v[u] = a.src[i + u * st];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a.dst[i + u * st] = d[u] + v[u];
+  }
+  for (; i < a.len; i += st) a.dst[i] += a.src[i];
 }
 
 }  // namespace
