@@ -315,6 +315,14 @@ __device__ __forceinline__ void backward_lu_stream2_body(SolveArgs<TW, TX> a, in
   const TX vjp = row ? x[size_t(jp) * n + r] : TX(0);
   const TX vjpp = row ? x[size_t(jpp) * n + r] : TX(0);
   TX s = row ? x[size_t(j) * n + r] : TX(0);
+  // B_j's first eight columns, in flight with the first D_j, E_j chunk
+  // (used after the D_j, E_j columns, as below; xb is read there)
+  TW bv0[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int cc = i < nx ? i : 0;
+    bv0[i] = gload(Bw, ob + unsigned(cc) * n);
+  }
   // D_j x_jp + E_j x_jpp, kChunk columns per memory latency
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     TW d[kChunk], e[kChunk];
@@ -331,7 +339,15 @@ __device__ __forceinline__ void backward_lu_stream2_body(SolveArgs<TW, TX> a, in
     }
   }
   // B_j xb, eight columns' loads in flight at a time
-  for (int c0 = 0; c0 < nx; c0 += 8) {
+  {
+    TX xv0[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) xv0[i] = x[xbrow + (i < nx ? i : 0)];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i < nx) s -= TX(keep_if(bv0[i], row)) * xv0[i];
+  }
+  for (int c0 = 8; c0 < nx; c0 += 8) {
     TW bv[8];
     TX xv[8];
 #pragma unroll
