@@ -148,6 +148,11 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
       const TX vjp = row ? x[size_t(jp) * n + lane] : TX(0);
       const TX vjpp = row ? x[size_t(jpp) * n + lane] : TX(0);
       TX s = row ? x[size_t(j) * n + lane] : TX(0);
+      // B_j's first two columns, in flight with the first D_j, E_j chunk
+      // (eight would cost this kernel two waves per SIMD)
+      TW bv0[2];
+  #pragma unroll
+      for (int i = 0; i < 2; ++i) bv0[i] = gload(Bj, rc + unsigned(i < nx ? i : 0) * n);
       // D_j, E_j columns kChunk at a time (one memory latency per chunk)
       for (int c0 = 0; c0 < n; c0 += kChunk) {
         TW d[kChunk], e[kChunk];
@@ -163,8 +168,12 @@ __device__ __forceinline__ void backward_lu_stream_body(SolveArgs<TW, TX> a, int
           if (c < n) s -= TX(d[i]) * TX(rlane(double(vjp), c)) + TX(e[i]) * TX(rlane(double(vjpp), c));
         }
       }
-      // B_j xb, eight columns' loads in flight at a time
-      for (int c0 = 0; c0 < nx; c0 += 8) {
+      // B_j xb: the two columns above, then eight columns' loads in flight
+      // at a time
+  #pragma unroll
+      for (int i = 0; i < 2; ++i)
+        if (i < nx) s -= TX(keep_if(bv0[i], row)) * x[xbrow + i];
+      for (int c0 = 2; c0 < nx; c0 += 8) {
         TW bv[8];
         TX xv[8];
   #pragma unroll
