@@ -324,14 +324,10 @@ __device__ __forceinline__ void backward_lu_stream2_body(SolveArgs<TW, TX> a, in
   const TX vjp = row ? x[size_t(jp) * n + r] : TX(0);
   const TX vjpp = row ? x[size_t(jpp) * n + r] : TX(0);
   TX s = row ? x[size_t(j) * n + r] : TX(0);
-  // B_j's first eight columns, in flight with the first D_j, E_j chunk
-  // (used after the D_j, E_j columns, as below; xb is read there)
-  TW bv0[8];
+  // B_j's first two columns, in flight with the first D_j, E_j chunk
+  TW bv0[2];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int cc = i < nx ? i : 0;
-    bv0[i] = gload(Bw, ob + unsigned(cc) * n);
-  }
+  for (int i = 0; i < 2; ++i) bv0[i] = gload(Bw, ob + unsigned(i < nx ? i : 0) * n);
   // D_j x_jp + E_j x_jpp, kChunk columns per memory latency
   for (int c0 = 0; c0 < n; c0 += kChunk) {
     TW d[kChunk], e[kChunk];
@@ -347,16 +343,12 @@ __device__ __forceinline__ void backward_lu_stream2_body(SolveArgs<TW, TX> a, in
       if (c < n) s -= TX(d[i]) * bc(vjp, c) + TX(e[i]) * bc(vjpp, c);
     }
   }
-  // B_j xb, eight columns' loads in flight at a time
-  {
-    TX xv0[8];
+  // B_j xb: the two columns above, then eight columns' loads in flight at
+  // a time
 #pragma unroll
-    for (int i = 0; i < 8; ++i) xv0[i] = x[xbrow + (i < nx ? i : 0)];
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-      if (i < nx) s -= TX(keep_if(bv0[i], row)) * xv0[i];
-  }
-  for (int c0 = 8; c0 < nx; c0 += 8) {
+  for (int i = 0; i < 2; ++i)
+    if (i < nx) s -= TX(keep_if(bv0[i], row)) * x[xbrow + i];
+  for (int c0 = 2; c0 < nx; c0 += 8) {
     TW bv[8];
     TX xv[8];
 #pragma unroll
