@@ -47,6 +47,21 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
   const TW* Tj = a.Tm + size_t(j) * tsize(n);
   const int r0 = lane, r1 = lane + 64;
   const bool v0 = r0 < n2, v1 = r1 < n2;
+  // factor columns kChunk at a time, from clamped addresses: one memory
+  // latency per chunk (rows past 2n hold garbage that is never used); the
+  // first chunk is in flight while the interchanges are composed and x is
+  // gathered
+  const int r0c = v0 ? r0 : 0, r1c = v1 ? r1 : 0;
+  TW t0[kChunk], t1[kChunk];
+  auto load_chunk = [&](int c0) {
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) {
+      const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n2;
+      t0[i] = gload(Tj, r0c + cc);
+      t1[i] = gload(Tj, r1c + cc);
+    }
+  };
+  load_chunk(0);
   // source rows after the interchanges (laswp over rows 1..n, 1-based ipiv)
   int src0 = r0, src1 = r1;
   const int pvl = lane < n ? a.perm[size_t(j) * n + lane] - 1 : 0;
@@ -62,9 +77,6 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
   auto in_row = [&](int r) { return r < n ? size_t(jp) * n + r : size_t(j) * n + (r - n); };
   auto out_row = [&](int r) { return r < n ? size_t(j) * n + r : size_t(jp) * n + (r - n); };
   const TW* Cj = a.C + size_t(j) * nr * n;
-  // factor columns kChunk at a time, from clamped addresses: one memory
-  // latency per chunk (rows past 2n hold garbage that is never used)
-  const int r0c = v0 ? r0 : 0, r1c = v1 ? r1 : 0;
   for (int q0 = 0; q0 < nrhs; q0 += QB) {
     TX x0[QB], x1[QB];
 #pragma unroll
@@ -74,13 +86,7 @@ __device__ __forceinline__ void forward_lu_stream_body(SolveArgs<TW, TX> a, int 
       x1[b] = v1 ? x[in_row(src1)] : TX(0);
     }
     for (int c0 = 0; c0 < n; c0 += kChunk) {
-      TW t0[kChunk], t1[kChunk];
-#pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
-        const unsigned cc = unsigned(c0 + i < n ? c0 + i : 0) * n2;
-        t0[i] = gload(Tj, r0c + cc);
-        t1[i] = gload(Tj, r1c + cc);
-      }
+      if (c0 > 0 || q0 > 0) load_chunk(c0);
 #pragma unroll
       for (int i = 0; i < kChunk; ++i) {
         const int c = c0 + i;
